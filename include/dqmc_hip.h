@@ -1,0 +1,143 @@
+/*
+ * dqmc_hip.h -- C ABI of the MI355X (gfx950) DQMC local-sweep library.
+ *
+ * Drop-in boundary for MonteCarlo.jl's DQMC hot path.  The reference keeps one chain's
+ * Green's-function stack in `DQMCStack` (src/flavors/DQMC/stack.jl:1-74) and runs the
+ * sweep through update(::LocalSweep, mc, model, field) (updates/local_updates.jl:82).
+ * A `HIPStack <: AbstractDQMCStack` on the Julia side owns one `dqmc_chain_t` and forwards
+ * each stack method to the entry point named below (ccall; plain pointers and sizes).
+ *
+ * Conventions
+ *   - Matrices are column-major FP64 with an explicit leading dimension `ld` (>= n).
+ *   - A block-diagonal (two-flavour, MagneticHirschField) matrix is exchanged as an
+ *     n x (flavors*n) array, block b in columns b*n .. b*n+n-1.
+ *   - The Hirsch field is an n x slices Int8 array of +-1 (fields.jl:329-330).
+ *   - Slice numbers are 1-based as in the reference; pivots are 0-based.
+ *   - Every entry point returns 0 on success and a negative DQMC_E* code on failure;
+ *     dqmc_last_error() describes the failure.  Numerical events (negative or tied
+ *     acceptance probabilities, propagation errors) never fail a call: they are counted
+ *     in dqmc_stats_t, as the reference records them in DQMCAnalysis.
+ *   - A chain handle is used by one host thread at a time; its work runs on its own
+ *     HIP stream on the device chosen at creation.
+ */
+#ifndef DQMC_HIP_H
+#define DQMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DQMC_OK 0
+#define DQMC_E_INVALID (-1)
+#define DQMC_E_HIP (-2)
+#define DQMC_E_STATE (-3)
+#define DQMC_E_ALLOC (-4)
+
+#define DQMC_PRECISION_FP64 0  /* everything in FP64 */
+#define DQMC_PRECISION_MIXED 1 /* FP32 wraps and rank-1 updates; FP64 UDT, recompute, detratio */
+
+typedef struct dqmc_chain_s* dqmc_chain_t;
+
+typedef struct {
+  int n_sites;      /* N = length(lattice(model)); the block size                          */
+  int flavors;      /* 1 = DensityHirschField, 2 = MagneticHirschField (fields.jl:378,427)  */
+  int slices;       /* M = DQMCParameters.slices (parameters.jl:117)                        */
+  int safe_mult;    /* DQMCParameters.safe_mult                                             */
+  double delta_tau; /* DQMCParameters.delta_tau                                             */
+  double alpha;     /* Hirsch coupling field.alpha (fields.jl:370-376, 419-425)             */
+  int precision;    /* DQMC_PRECISION_*                                                     */
+  int check_propagation_error; /* DQMCParameters.check_propagation_error                   */
+  int delay_depth;  /* delayed-update depth, 0 = automatic                                  */
+  int device;       /* HIP device ordinal                                                   */
+} dqmc_config_t;
+
+typedef struct {
+  int64_t accepted;             /* accepted local flips (sum of sweep_spatial returns)       */
+  int64_t proposed;             /* proposals                                                 */
+  int64_t uniforms_used;        /* uniforms consumed (one per proposal with p <= 1)          */
+  int64_t negative_probability; /* proposals with p < 0 (local_updates.jl:41-47)             */
+  int64_t ties;                 /* |u - p| < 1e-12, or 1 < p < 1 + 1e-12                     */
+  int64_t stream_exhausted;     /* supplied uniform stream ran out (then u = 1, reject)      */
+  int64_t propagation_error_count; /* stabilisations with max|dG| > 1e-7 (stack.jl:644-654) */
+  int64_t stabilizations;       /* stabilisations checked                                    */
+  int64_t sweeps;               /* completed sweeps                                          */
+  int64_t delayed_flushes;      /* rank-k flushes of the delayed update                      */
+  double propagation_error_max; /* max over stabilisations of max|G_wrapped - G_scratch|    */
+  double propagation_error_last;
+} dqmc_stats_t;
+
+/* DQMCStack(field, model, checkerboard=false) + initialize_stack (stack.jl:113-206).
+ * Allocates every device buffer of the chain (footprint: dqmc_footprint_bytes).        */
+int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out);
+int dqmc_destroy(dqmc_chain_t h);
+const char* dqmc_last_error(dqmc_chain_t h); /* h may be NULL: last creation error */
+int dqmc_footprint_bytes(const dqmc_config_t* cfg, int64_t* bytes);
+
+/* init_hopping_matrices (stack.jl:209-249): exp(-+dtau T/2), exp(-+dtau T) of the N x N
+ * hopping matrix (hopping_matrix, HubbardModel.jl:112-124), computed on the device.    */
+int dqmc_set_hopping(dqmc_chain_t h, const double* T, int ld);
+/* Same, with the four exponentials supplied (e.g. the reference's own fallback_exp).   */
+int dqmc_set_hopping_exp(dqmc_chain_t h, const double* exp_mhalf, const double* exp_phalf,
+                         const double* exp_m, const double* exp_p, int ld);
+int dqmc_get_hopping_exp(dqmc_chain_t h, double* exp_mhalf, double* exp_phalf, double* exp_m,
+                         double* exp_p, int ld);
+
+/* field.conf (fields.jl:329-334), N x M Int8 +-1. */
+int dqmc_set_field(dqmc_chain_t h, const int8_t* conf, int ld);
+int dqmc_get_field(dqmc_chain_t h, int8_t* conf, int ld);
+
+/* Randomness of the Metropolis test `p > 1 || rand() < p` (local_updates.jl:53): a
+ * uniform is consumed only when p <= 1.  Either a supplied stream (copied to the
+ * device) or the counter stream u_k = SplitMix64(seed, k) >> 11 * 2^-53 from `position`. */
+int dqmc_set_uniforms(dqmc_chain_t h, const double* u, int64_t count);
+int dqmc_set_seed(dqmc_chain_t h, uint64_t seed, uint64_t position);
+/* Optional per-proposal log (accept flag and p) for parity checks. */
+int dqmc_set_decision_log(dqmc_chain_t h, int64_t capacity);
+int dqmc_get_decision_log(dqmc_chain_t h, uint8_t* accepted, double* p, int64_t capacity,
+                          int64_t* length);
+
+/* reverse_build_stack + propagate, as initialize_run does (DQMC.jl:178-179).           */
+int dqmc_build_stack(dqmc_chain_t h);
+/* build_stack (stack.jl:257-281) + propagate.                                          */
+int dqmc_build_stack_forward(dqmc_chain_t h);
+/* reverse_build_stack (stack.jl:284-308) alone.                                        */
+int dqmc_reverse_build_stack(dqmc_chain_t h);
+/* propagate (stack.jl:605-730).                                                        */
+int dqmc_propagate(dqmc_chain_t h);
+/* sweep_spatial (local_updates.jl:23-60) at the current slice; *accepted may be NULL.  */
+int dqmc_sweep_spatial(dqmc_chain_t h, int64_t* accepted);
+/* nsweeps x local_sweep (local_updates.jl:7-14): 2M x (sweep_spatial; propagate).      */
+int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted);
+
+/* mc.stack.greens (physical = 0) or greens(mc) = e^{dtau T/2} G e^{-dtau T/2}
+ * (physical = 1, greens.jl:87-125); n x (flavors*n).                                   */
+int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical);
+int dqmc_set_greens(dqmc_chain_t h, const double* G, int ld);
+/* calculate_greens(mc, slice, output) from scratch (stack.jl:525-583); does not touch
+ * the chain's current G.                                                               */
+int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* G, int ld);
+
+int dqmc_get_state(dqmc_chain_t h, int* current_slice, int* direction, int* current_range);
+int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* stats);
+/* max|G_wrapped - G_scratch| at every stabilisation since the last call (mixed precision:
+ * FP32-propagated G against the FP64 recompute).                                       */
+int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, int64_t* length);
+int dqmc_synchronize(dqmc_chain_t h);
+
+/* Device linear algebra on host arrays (unit parity of the kernels).
+ *   udt_AVX_pivot! (UDT.jl:216-334), rdivp! (real.jl:198-226),
+ *   calculate_greens_AVX! (stack.jl:442-496), vmul! (real.jl:7-102).                   */
+int dqmc_la_udt(int n, const double* X, int apply_pivot, double* U, double* D, double* T,
+                int32_t* pivot);
+int dqmc_la_rdivp(int n, double* A, const double* T, const int32_t* pivot);
+int dqmc_la_calc_greens_udt(int n, const double* Ul, const double* Dl, const double* Tl,
+                            const double* Ur, const double* Dr, const double* Tr, double* G);
+int dqmc_la_gemm(int n, int trans_a, int trans_b, const double* A, const double* B, double* C,
+                 int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DQMC_HIP_H */

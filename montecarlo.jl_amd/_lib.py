@@ -1,0 +1,301 @@
+"""ctypes binding of libdqmc_hip.so (the C ABI declared in include/dqmc_hip.h).
+
+The library is the only compute path: if it is not built, or no GPU is visible, the
+calls raise instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdqmc_hip.so")
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+class DQMCError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("n_sites", C.c_int), ("flavors", C.c_int), ("slices", C.c_int),
+                ("safe_mult", C.c_int), ("delta_tau", C.c_double), ("alpha", C.c_double),
+                ("precision", C.c_int), ("check_propagation_error", C.c_int),
+                ("delay_depth", C.c_int), ("device", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("accepted", C.c_int64), ("proposed", C.c_int64), ("uniforms_used", C.c_int64),
+                ("negative_probability", C.c_int64), ("ties", C.c_int64),
+                ("stream_exhausted", C.c_int64), ("propagation_error_count", C.c_int64),
+                ("stabilizations", C.c_int64), ("sweeps", C.c_int64),
+                ("delayed_flushes", C.c_int64), ("propagation_error_max", C.c_double),
+                ("propagation_error_last", C.c_double)]
+
+
+EXPORTED = [
+    "dqmc_create", "dqmc_destroy", "dqmc_last_error", "dqmc_footprint_bytes",
+    "dqmc_set_hopping", "dqmc_set_hopping_exp", "dqmc_get_hopping_exp",
+    "dqmc_set_field", "dqmc_get_field", "dqmc_set_uniforms", "dqmc_set_seed",
+    "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
+    "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
+    "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_get_greens", "dqmc_set_greens",
+    "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
+    "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
+    "dqmc_la_calc_greens_udt", "dqmc_la_gemm",
+]
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_i8 = np.ctypeslib.ndpointer(dtype=np.int8, flags="F_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def load():
+    """Load the library (raises DQMCError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DQMCError(f"{LIB_PATH} is not built; run montecarlo.jl_amd/build.py "
+                        "(the HIP library is the only compute path)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, i64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    sig = {
+        "dqmc_create": [C.POINTER(Config), C.POINTER(vp)],
+        "dqmc_destroy": [vp],
+        "dqmc_footprint_bytes": [C.POINTER(Config), C.POINTER(i64)],
+        "dqmc_set_hopping": [vp, _dp, i],
+        "dqmc_set_hopping_exp": [vp, _dp, _dp, _dp, _dp, i],
+        "dqmc_get_hopping_exp": [vp, _dp, _dp, _dp, _dp, i],
+        "dqmc_set_field": [vp, _i8, i],
+        "dqmc_get_field": [vp, _i8, i],
+        "dqmc_set_uniforms": [vp, C.POINTER(d), i64],
+        "dqmc_set_seed": [vp, C.c_uint64, C.c_uint64],
+        "dqmc_set_decision_log": [vp, i64],
+        "dqmc_get_decision_log": [vp, vp, vp, i64, C.POINTER(i64)],
+        "dqmc_build_stack": [vp], "dqmc_build_stack_forward": [vp],
+        "dqmc_reverse_build_stack": [vp], "dqmc_propagate": [vp],
+        "dqmc_sweep_spatial": [vp, C.POINTER(i64)],
+        "dqmc_sweep": [vp, i, C.POINTER(i64)],
+        "dqmc_get_greens": [vp, _dp, i, i],
+        "dqmc_set_greens": [vp, _dp, i],
+        "dqmc_calculate_greens": [vp, i, _dp, i],
+        "dqmc_get_state": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
+        "dqmc_get_stats": [vp, C.POINTER(Stats)],
+        "dqmc_get_stabilization_log": [vp, vp, i64, C.POINTER(i64)],
+        "dqmc_synchronize": [vp],
+        "dqmc_la_udt": [i, _dp, i, _dp, _dp, _dp, _i32],
+        "dqmc_la_rdivp": [i, _dp, _dp, _i32],
+        "dqmc_la_calc_greens_udt": [i, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
+        "dqmc_la_gemm": [i, i, i, _dp, _dp, _dp, i],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    L.dqmc_last_error.argtypes = [vp]
+    L.dqmc_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc: int, h=None, what: str = ""):
+    if rc != 0:
+        msg = load().dqmc_last_error(h)
+        raise DQMCError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def footprint_bytes(n: int, flavors: int, slices: int, safe_mult: int, precision: str = "fp64") -> int:
+    cfg = Config(n, flavors, slices, safe_mult, 0.1, 0.0, PRECISION[precision], 1, 0, 0)
+    out = C.c_int64()
+    _check(load().dqmc_footprint_bytes(C.byref(cfg), C.byref(out)), None, "dqmc_footprint_bytes")
+    return out.value
+
+
+class HIPChain:
+    """One DQMC chain resident on the GPU (C-ABI handle).  Green's functions are
+    exchanged as (flavors, n, n) arrays."""
+
+    def __init__(self, n: int, flv: int, slices: int, safe_mult: int, delta_tau: float,
+                 alpha: float, check_propagation_error: bool = True, precision: str = "fp64",
+                 delay_depth: int = 0, device: int = 0):
+        L = load()
+        self.n, self.flv, self.M, self.safe_mult = n, flv, slices, safe_mult
+        self.delta_tau, self.alpha, self.precision = delta_tau, alpha, precision
+        cfg = Config(n, flv, slices, safe_mult, delta_tau, alpha, PRECISION[precision],
+                     int(check_propagation_error), delay_depth, device)
+        h = C.c_void_p()
+        _check(L.dqmc_create(C.byref(cfg), C.byref(h)), None, "dqmc_create")
+        self._h = h
+        self._u = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dqmc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        _check(rc, self._h, what)
+
+    def set_hopping(self, T):
+        T = _f(T)
+        self._c(load().dqmc_set_hopping(self._h, T, T.shape[0]), "dqmc_set_hopping")
+
+    def set_hopping_exp(self, eTmh, eTph, eT2m, eT2p):
+        a = [_f(x) for x in (eTmh, eTph, eT2m, eT2p)]
+        self._c(load().dqmc_set_hopping_exp(self._h, *a, self.n), "dqmc_set_hopping_exp")
+
+    def get_hopping_exp(self):
+        out = [np.zeros((self.n, self.n), order="F") for _ in range(4)]
+        self._c(load().dqmc_get_hopping_exp(self._h, *out, self.n), "dqmc_get_hopping_exp")
+        return out
+
+    def set_field(self, conf):
+        c = _f(conf, np.int8)
+        self._c(load().dqmc_set_field(self._h, c, c.shape[0]), "dqmc_set_field")
+
+    def get_field(self):
+        c = np.zeros((self.n, self.M), dtype=np.int8, order="F")
+        self._c(load().dqmc_get_field(self._h, c, self.n), "dqmc_get_field")
+        return c
+
+    def set_uniforms(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._u = u
+        self._c(load().dqmc_set_uniforms(self._h, u.ctypes.data_as(C.POINTER(C.c_double)), len(u)),
+                "dqmc_set_uniforms")
+
+    def set_seed(self, seed: int, position: int = 0):
+        self._c(load().dqmc_set_seed(self._h, C.c_uint64(seed), C.c_uint64(position)), "dqmc_set_seed")
+
+    def set_log(self, cap: int):
+        self._c(load().dqmc_set_decision_log(self._h, cap), "dqmc_set_decision_log")
+        self._logcap = cap
+
+    def get_log(self):
+        length = C.c_int64()
+        self._c(load().dqmc_get_decision_log(self._h, None, None, 0, C.byref(length)), "log")
+        n = length.value
+        dec = np.zeros(max(n, 1), dtype=np.uint8)
+        p = np.zeros(max(n, 1), dtype=np.float64)
+        self._c(load().dqmc_get_decision_log(self._h, dec.ctypes.data, p.ctypes.data, n,
+                                             C.byref(length)), "log")
+        return dec[:n], p[:n]
+
+    def build_stack(self):
+        self._c(load().dqmc_build_stack(self._h), "dqmc_build_stack")
+
+    def build_stack_forward(self):
+        self._c(load().dqmc_build_stack_forward(self._h), "dqmc_build_stack_forward")
+
+    def reverse_build_stack(self):
+        self._c(load().dqmc_reverse_build_stack(self._h), "dqmc_reverse_build_stack")
+
+    def propagate(self):
+        self._c(load().dqmc_propagate(self._h), "dqmc_propagate")
+
+    def sweep_spatial(self) -> int:
+        a = C.c_int64()
+        self._c(load().dqmc_sweep_spatial(self._h, C.byref(a)), "dqmc_sweep_spatial")
+        return a.value
+
+    def sweep(self, nsweeps: int = 1) -> int:
+        a = C.c_int64()
+        self._c(load().dqmc_sweep(self._h, nsweeps, C.byref(a)), "dqmc_sweep")
+        return a.value
+
+    def _blocks(self, G):
+        return np.stack([G[:, b * self.n:(b + 1) * self.n] for b in range(self.flv)])
+
+    def greens(self, physical: bool = False) -> np.ndarray:
+        G = np.zeros((self.n, self.n * self.flv), order="F")
+        self._c(load().dqmc_get_greens(self._h, G, self.n, int(physical)), "dqmc_get_greens")
+        return self._blocks(G)
+
+    def set_greens(self, G):
+        flat = np.asfortranarray(np.concatenate(list(G), axis=1))
+        self._c(load().dqmc_set_greens(self._h, flat, self.n), "dqmc_set_greens")
+
+    def calculate_greens(self, slice_: int) -> np.ndarray:
+        G = np.zeros((self.n, self.n * self.flv), order="F")
+        self._c(load().dqmc_calculate_greens(self._h, slice_, G, self.n), "dqmc_calculate_greens")
+        return self._blocks(G)
+
+    def state(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._c(load().dqmc_get_state(self._h, C.byref(a), C.byref(b), C.byref(c)), "state")
+        return a.value, b.value, c.value
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._c(load().dqmc_get_stats(self._h, C.byref(s)), "dqmc_get_stats")
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def counters(self) -> dict:
+        s = self.stats()
+        return {"accepted": s["accepted"], "proposed": s["proposed"],
+                "uniforms_used": s["uniforms_used"], "negative_prob": s["negative_probability"],
+                "ties": s["ties"], "prop_err_count": s["propagation_error_count"],
+                "stabilizations": s["stabilizations"], "stream_exhausted": s["stream_exhausted"],
+                "prop_err_max": s["propagation_error_max"],
+                "prop_err_last": s["propagation_error_last"]}
+
+    def stabilization_log(self) -> np.ndarray:
+        length = C.c_int64()
+        L = load()
+        buf = np.zeros(1 << 16, dtype=np.float64)
+        self._c(L.dqmc_get_stabilization_log(self._h, buf.ctypes.data, len(buf), C.byref(length)),
+                "dqmc_get_stabilization_log")
+        return buf[:min(length.value, len(buf))].copy()
+
+    def synchronize(self):
+        self._c(load().dqmc_synchronize(self._h), "dqmc_synchronize")
+
+
+# ---- kernel-level entry points -----------------------------------------------------------
+def la_udt(X, apply_pivot: bool = True):
+    n = X.shape[0]
+    U = np.zeros((n, n), order="F")
+    D = np.zeros((n, 1), order="F")
+    T = np.zeros((n, n), order="F")
+    piv = np.zeros(n, dtype=np.int32)
+    _check(load().dqmc_la_udt(n, _f(X), int(apply_pivot), U, D, T, piv), None, "dqmc_la_udt")
+    return U, D[:, 0].copy(), T, piv
+
+
+def la_rdivp(A, T, pivot):
+    n = A.shape[0]
+    out = _f(A).copy(order="F")
+    _check(load().dqmc_la_rdivp(n, out, _f(T), np.ascontiguousarray(pivot, dtype=np.int32)),
+           None, "dqmc_la_rdivp")
+    return out
+
+
+def la_calc_greens_udt(Ul, Dl, Tl, Ur, Dr, Tr):
+    n = Ul.shape[0]
+    G = np.zeros((n, n), order="F")
+    args = [_f(Ul), _f(np.reshape(Dl, (n, 1))), _f(Tl), _f(Ur), _f(np.reshape(Dr, (n, 1))), _f(Tr)]
+    _check(load().dqmc_la_calc_greens_udt(n, *args, G), None, "dqmc_la_calc_greens_udt")
+    return G
+
+
+def la_gemm(A, B, ta=False, tb=False, precision: str = "fp64"):
+    n = A.shape[0]
+    Cm = np.zeros((n, n), order="F")
+    _check(load().dqmc_la_gemm(n, int(ta), int(tb), _f(A), _f(B), Cm, PRECISION[precision]),
+           None, "dqmc_la_gemm")
+    return Cm

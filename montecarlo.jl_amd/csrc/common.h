@@ -1,0 +1,159 @@
+// Shared declarations of the gfx950 DQMC library (kernels + host orchestration).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dqmc {
+
+// A diagonal operand d[k] given either as a vector, or generated from a Hirsch
+// field column: d[k] = conf[k] > 0 ? pos : neg  (exp(+-alpha) tables, so the
+// interaction matrix exp(power * alpha * x) never needs its own buffer).
+struct DiagSpec {
+  const double* vec;     // nullptr: not a vector
+  const int8_t* conf;    // nullptr: not a field column
+  double pos, neg;       // values for conf > 0 / conf < 0
+  int invert;            // use 1 / vec[k]
+};
+
+// Row held in accumulator element r of lane `lane` for the 16x16x4 MFMA tiles
+// (column = lane % 16 for both):
+//   v_mfma_f64_16x16x4f64: row = lane / 16 + 4 r
+//   v_mfma_f32_16x16x4f32: row = 4 (lane / 16) + r
+template <typename T> __device__ __forceinline__ int mfma_row(int lane, int r);
+template <> __device__ __forceinline__ int mfma_row<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
+template <> __device__ __forceinline__ int mfma_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+__host__ __device__ inline DiagSpec diag_none() { return DiagSpec{nullptr, nullptr, 1.0, 1.0, 0}; }
+__host__ __device__ inline DiagSpec diag_vec(const double* v, int inv = 0) {
+  return DiagSpec{v, nullptr, 1.0, 1.0, inv};
+}
+__host__ __device__ inline DiagSpec diag_field(const int8_t* c, double pos, double neg) {
+  return DiagSpec{nullptr, c, pos, neg, 0};
+}
+__host__ __device__ inline bool diag_active(const DiagSpec& d) { return d.vec || d.conf; }
+__device__ inline double diag_at(const DiagSpec& d, int k) {
+  if (d.vec) return d.invert ? 1.0 / d.vec[k] : d.vec[k];
+  if (d.conf) return d.conf[k] > 0 ? d.pos : d.neg;
+  return 1.0;
+}
+
+// One matrix product of a batched launch:
+//   C = alpha * diag(dl) * op(A) * diag(dmid) * op(B) * diag(dr) + beta * C + diag(dadd)
+template <typename T>
+struct GemmItem {
+  const T* A;
+  const T* B;
+  T* C;
+  int lda, ldb, ldc;
+  int M, N, K;
+  DiagSpec dl, dmid, dr, dadd;
+};
+
+constexpr int kMaxGemmItems = 4;
+
+template <typename T>
+struct GemmBatch {
+  GemmItem<T> it[kMaxGemmItems];
+  int count;
+  int transA, transB;
+  double alpha, beta;
+};
+
+// Launchers (gemm.hip)
+hipError_t launch_gemm_f64(const GemmBatch<double>& b, hipStream_t s);
+hipError_t launch_gemm_f32(const GemmBatch<float>& b, hipStream_t s);
+
+// Pivoted UDT of a batch of n x n matrices (qr.hip).
+struct UdtItem {
+  double* A;        // input, destroyed (Householder vectors / R)
+  double* U;        // orthogonal factor
+  double* D;        // |diag R| (0 -> 1)
+  double* T;        // D^-1 R, pivoted (apply_pivot) or "dirty upper" (may alias A then)
+  int* pivot;       // column permutation, 0-based
+  double* tau;      // n scratch
+  double* tfac;     // (n + 16) * 16 scratch (compact-WY T factors, 16 x 16 per panel start)
+  double* swap;     // 32 * n scratch (column moves)
+};
+constexpr int kMaxUdtItems = 4;
+struct UdtBatch {
+  UdtItem it[kMaxUdtItems];
+  int count;
+  int n;
+  int apply_pivot;
+};
+hipError_t launch_udt(const UdtBatch& b, hipStream_t s);
+
+// rdivp: A <- A[:, pivot] * triu(T)^-1 with O as n*n scratch (qr.hip).
+struct RdivpItem {
+  double* A;
+  const double* T;
+  double* O;
+  const int* pivot;
+};
+struct RdivpBatch {
+  RdivpItem it[kMaxUdtItems];
+  int count;
+  int n;
+};
+hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s);
+
+// Elementwise helpers (util.hip)
+hipError_t launch_set_identity(double* A, int n, int nmat, hipStream_t s);
+hipError_t launch_set_identity_f32(float* A, int n, int nmat, hipStream_t s);
+hipError_t launch_fill(double* x, double v, int64_t count, hipStream_t s);
+hipError_t launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
+hipError_t launch_f64_to_f32(float* dst, const double* src, int64_t count, hipStream_t s);
+hipError_t launch_f32_to_f64(double* dst, const float* src, int64_t count, hipStream_t s);
+hipError_t launch_symmetrize(double* A, int n, hipStream_t s);
+// out[slot] = max |a - b| over count elements (a may be f32 when a32 != nullptr)
+hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b, int64_t count,
+                             double* out, hipStream_t s);
+hipError_t launch_scale_add_identity(double* A, double scale, double add_diag, int n, hipStream_t s);
+hipError_t launch_axpy_matrix(double* Y, const double* X, double a, int64_t count, hipStream_t s);
+hipError_t launch_norm1(const double* A, int n, double* out, hipStream_t s);
+// out = in * diag(d) for nmat consecutive n x n matrices (d: nmat consecutive vectors)
+hipError_t launch_scale_cols(double* out, const double* in, const double* d, int n, int nmat,
+                             hipStream_t s);
+
+// One-time kernel attribute setup (dynamic LDS above 64 KB).
+hipError_t qr_kernels_init();
+hipError_t local_kernels_init();
+
+// Local update (local.hip)
+struct ChainCounters {
+  long long u_pos;        // uniforms consumed
+  long long accepted;
+  long long proposed;
+  long long negative;
+  long long ties;
+  long long exhausted;
+  long long log_len;
+  long long flushes;
+};
+
+template <typename GT>
+struct LocalArgs {
+  GT* G;                  // flv blocks, n x n, ld = n
+  int n, flv;
+  int8_t* conf;           // conf column of the current slice
+  int slice;              // 1-based, for the log
+  double d_first[2];      // Delta block 1, index (x > 0)
+  double d_second[2];     // Delta block 2 (magnetic)
+  double bweight[2];      // exp(-dE_boson) density, index (x > 0)
+  GT* ubuf;               // flv * n * kmax (global spill for U columns)
+  GT* vbuf;               // flv * n * kmax
+  int kmax;
+  int lds_uv;             // U/V held in LDS
+  const double* uniforms; // supplied stream (nullptr: counter stream)
+  long long n_uniforms;
+  unsigned long long seed;
+  ChainCounters* ctr;
+  unsigned char* dec_log;
+  double* p_log;
+  long long log_cap;
+};
+hipError_t launch_local_f64(const LocalArgs<double>& a, hipStream_t s);
+hipError_t launch_local_f32(const LocalArgs<float>& a, hipStream_t s);
+int local_lds_kmax(int n, int flv, int elem_bytes);
+
+}  // namespace dqmc

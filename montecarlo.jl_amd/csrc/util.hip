@@ -1,0 +1,194 @@
+// Elementwise and reduction helpers: identity / copy / precision conversion, the
+// symmetrisation of the hopping exponentials and the max|G_wrapped - G_scratch|
+// reduction of the propagation-error check (reference stack.jl:634-654, 693-709).
+#include "common.h"
+
+namespace dqmc {
+
+static inline dim3 grid_for(int64_t count, int bs = 256) {
+  int64_t g = (count + bs - 1) / bs;
+  if (g > 65535 * 8) g = 65535 * 8;
+  return dim3((unsigned)g);
+}
+
+template <typename T>
+__global__ void set_identity_kernel(T* A, int n, int64_t total) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = i % ((int64_t)n * n);
+    const int r = (int)(e % n), c = (int)(e / n);
+    A[i] = (r == c) ? T(1) : T(0);
+  }
+}
+hipError_t launch_set_identity(double* A, int n, int nmat, hipStream_t s) {
+  int64_t total = (int64_t)n * n * nmat;
+  hipLaunchKernelGGL(set_identity_kernel<double>, grid_for(total), dim3(256), 0, s, A, n, total);
+  return hipGetLastError();
+}
+hipError_t launch_set_identity_f32(float* A, int n, int nmat, hipStream_t s) {
+  int64_t total = (int64_t)n * n * nmat;
+  hipLaunchKernelGGL(set_identity_kernel<float>, grid_for(total), dim3(256), 0, s, A, n, total);
+  return hipGetLastError();
+}
+
+__global__ void fill_kernel(double* x, double v, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+hipError_t launch_fill(double* x, double v, int64_t count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_kernel, grid_for(count), dim3(256), 0, s, x, v, count);
+  return hipGetLastError();
+}
+
+__global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+hipError_t launch_copy(double* dst, const double* src, int64_t count, hipStream_t s) {
+  if (count <= 0 || dst == src) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel, grid_for(count), dim3(256), 0, s, dst, src, count);
+  return hipGetLastError();
+}
+
+__global__ void f64_to_f32_kernel(float* __restrict__ dst, const double* __restrict__ src, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+__global__ void f32_to_f64_kernel(double* __restrict__ dst, const float* __restrict__ src, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (double)src[i];
+}
+hipError_t launch_f64_to_f32(float* dst, const double* src, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(f64_to_f32_kernel, grid_for(count), dim3(256), 0, s, dst, src, count);
+  return hipGetLastError();
+}
+hipError_t launch_f32_to_f64(double* dst, const float* src, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(f32_to_f64_kernel, grid_for(count), dim3(256), 0, s, dst, src, count);
+  return hipGetLastError();
+}
+
+// A <- (A + A') / 2
+__global__ void symmetrize_kernel(double* A, int n) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    if (r < c) {
+      const double v = 0.5 * (A[r + (size_t)c * n] + A[c + (size_t)r * n]);
+      A[r + (size_t)c * n] = v;
+      A[c + (size_t)r * n] = v;
+    }
+  }
+}
+hipError_t launch_symmetrize(double* A, int n, hipStream_t s) {
+  hipLaunchKernelGGL(symmetrize_kernel, grid_for((int64_t)n * n), dim3(256), 0, s, A, n);
+  return hipGetLastError();
+}
+
+// A <- scale * A + add_diag * I
+__global__ void scale_add_identity_kernel(double* A, double scale, double add, int n) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    A[e] = scale * A[e] + (r == c ? add : 0.0);
+  }
+}
+hipError_t launch_scale_add_identity(double* A, double scale, double add_diag, int n, hipStream_t s) {
+  hipLaunchKernelGGL(scale_add_identity_kernel, grid_for((int64_t)n * n), dim3(256), 0, s, A, scale,
+                     add_diag, n);
+  return hipGetLastError();
+}
+
+__global__ void axpy_kernel(double* __restrict__ Y, const double* __restrict__ X, double a, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Y[i] += a * X[i];
+}
+hipError_t launch_axpy_matrix(double* Y, const double* X, double a, int64_t count, hipStream_t s) {
+  hipLaunchKernelGGL(axpy_kernel, grid_for(count), dim3(256), 0, s, Y, X, a, count);
+  return hipGetLastError();
+}
+
+__device__ inline double wave_max(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Single-workgroup reductions: results are deterministic (fixed reduction tree).
+__global__ __launch_bounds__(1024) void maxabsdiff_kernel(const double* __restrict__ a,
+                                                          const float* __restrict__ a32,
+                                                          const double* __restrict__ b, int64_t count,
+                                                          double* out) {
+  __shared__ double part[16];
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += blockDim.x) {
+    const double av = a32 ? (double)a32[i] : a[i];
+    const double d = fabs(av - b[i]);
+    m = (d > m || d != d) ? d : m;
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = (part[w] > r || part[w] != part[w]) ? part[w] : r;
+    *out = r;
+  }
+}
+hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b, int64_t count,
+                             double* out, hipStream_t s) {
+  hipLaunchKernelGGL(maxabsdiff_kernel, dim3(1), dim3(1024), 0, s, a, a32, b, count, out);
+  return hipGetLastError();
+}
+
+// max column sum |A| (1-norm), used to pick the scaling of the matrix exponential
+__global__ __launch_bounds__(1024) void norm1_kernel(const double* __restrict__ A, int n, double* out) {
+  __shared__ double part[16];
+  double m = 0.0;
+  for (int c = threadIdx.x; c < n; c += blockDim.x) {
+    double s = 0.0;
+    for (int r = 0; r < n; r++) s += fabs(A[r + (size_t)c * n]);
+    m = fmax(m, s);
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = fmax(r, part[w]);
+    *out = r;
+  }
+}
+hipError_t launch_norm1(const double* A, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(norm1_kernel, dim3(1), dim3(1024), 0, s, A, n, out);
+  return hipGetLastError();
+}
+
+__global__ void scale_cols_kernel(double* __restrict__ out, const double* __restrict__ in,
+                                  const double* __restrict__ d, int n, int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = e / nn;
+    const int c = (int)((e - m * nn) / n);
+    out[e] = in[e] * d[m * n + c];
+  }
+}
+hipError_t launch_scale_cols(double* out, const double* in, const double* d, int n, int nmat,
+                             hipStream_t s) {
+  const int64_t total = (int64_t)n * n * nmat;
+  hipLaunchKernelGGL(scale_cols_kernel, grid_for(total), dim3(256), 0, s, out, in, d, n, total);
+  return hipGetLastError();
+}
+
+}  // namespace dqmc

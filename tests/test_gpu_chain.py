@@ -1,0 +1,156 @@
+"""Chain-level parity of the GPU sweep against the CPU oracle (same inputs, same uniforms).
+
+Reference path: reverse_build_stack/propagate (stack.jl:284-308, 605-730), calculate_greens
+(stack.jl:525-583), sweep_spatial (local_updates.jl:23-60).
+Tolerances: G within 1e-10 relative (max-norm) in FP64; accept sequence and field bit-exact
+except proposals whose |u - p| is within the tie band (counted by both sides).
+"""
+import numpy as np
+import pytest
+
+from conftest import make_chain, rel_maxdiff
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (L, U_ref, beta, mu): U_ref > 0 attractive (density field, 1 flavour), < 0 repulsive
+    (4, 4.0, 4.0, 0.0),     # attractive 4x4, beta=4 (M=40): the CPU-path configuration
+    (4, -4.0, 2.0, 0.0),    # repulsive 4x4 (magnetic, 2 flavours)
+    (6, 4.0, 3.0, -0.5),    # attractive 6x6 with mu
+    (3, -2.0, 1.5, 0.0),    # odd lattice, M=15 (uneven chunks)
+]
+
+
+def pair(pkg, hip, oracle, L, U, beta, mu, seed=1234, **kw):
+    g, model, par, field = make_chain(hip.HIPChain, pkg, L, U, beta, mu=mu, seed=seed, **kw)
+    o, *_ = make_chain(oracle.OracleChain, pkg, L, U, beta, mu=mu, seed=seed)
+    T = model.hopping_matrix()
+    o.set_hopping(T)
+    g.set_hopping_exp(*o.get_hopping_exp())
+    g.set_field(field.conf)
+    o.set_field(field.conf)
+    return g, o, model, par, field
+
+
+@pytest.mark.parametrize("L,U,beta,mu", CASES)
+def test_build_stack_greens(pkg, hip, oracle, L, U, beta, mu):
+    g, o, *_ = pair(pkg, hip, oracle, L, U, beta, mu)
+    g.build_stack()
+    o.build_stack()
+    assert g.state() == o.state()
+    assert rel_maxdiff(g.greens(), o.greens()) < 1e-10
+    assert rel_maxdiff(g.greens(physical=True), o.greens(physical=True)) < 1e-10
+
+
+@pytest.mark.parametrize("L,U,beta,mu", CASES[:2])
+def test_forward_build_equals_reverse_build(pkg, hip, oracle, L, U, beta, mu):
+    """build_stack + M propagates == reverse_build_stack + propagate (flavortests_DQMC.jl:313-342)."""
+    g, o, model, par, field = pair(pkg, hip, oracle, L, U, beta, mu)
+    g.build_stack_forward()
+    for _ in range(par.slices):
+        g.propagate()
+    g2, *_ = pair(pkg, hip, oracle, L, U, beta, mu)
+    g2.build_stack()
+    assert g.state() == g2.state()
+    assert rel_maxdiff(g.greens(), g2.greens()) < 1e-10
+
+
+@pytest.mark.parametrize("L,U,beta,mu", CASES)
+def test_calculate_greens_from_scratch(pkg, hip, oracle, L, U, beta, mu):
+    g, o, model, par, field = pair(pkg, hip, oracle, L, U, beta, mu)
+    for sl in sorted({0, 1, par.slices // 3, par.slices - 1, par.slices}):
+        assert rel_maxdiff(g.calculate_greens(sl), o.calculate_greens(sl)) < 1e-10, sl
+
+
+def test_hopping_exponentials_on_device(pkg, hip, oracle):
+    """exp(-+dtau T/2), exp(-+dtau T) computed on the GPU == eigen-decomposition oracle."""
+    for lat in (pkg.SquareLattice(4), pkg.Honeycomb(3), pkg.SquareLattice(2)):
+        model = pkg.HubbardModel(l=lat, U=1.0, mu=0.3)
+        n = len(lat)
+        g = hip.HIPChain(n, 1, 10, 10, 0.1, 0.5)
+        o = oracle.OracleChain(n, 1, 10, 10, 0.1, 0.5)
+        T = model.hopping_matrix()
+        g.set_hopping(T)
+        o.set_hopping(T)
+        for a, b in zip(g.get_hopping_exp(), o.get_hopping_exp()):
+            assert np.abs(a - b).max() < 1e-14
+            assert np.array_equal(a, a.T)
+
+
+@pytest.mark.parametrize("L,U,beta,mu", CASES)
+@pytest.mark.parametrize("stream", ["supplied", "counter"])
+def test_one_sweep_accept_sequence(pkg, hip, oracle, L, U, beta, mu, stream):
+    g, o, model, par, field = pair(pkg, hip, oracle, L, U, beta, mu)
+    n, M = len(model.l), par.slices
+    cap = 2 * n * M
+    if stream == "supplied":
+        u = np.random.default_rng(99).random(cap)
+        g.set_uniforms(u)
+        o.set_uniforms(u)
+    else:
+        g.set_seed(2024)
+        o.set_seed(2024)
+    g.set_log(cap)
+    o.set_log(cap)
+    g.build_stack()
+    o.build_stack()
+    ag = g.sweep(1)
+    ao = o.sweep(1)
+    dg, pg = g.get_log()
+    do, po = o.get_log()
+    cg, co = g.counters(), o.counters()
+    assert len(dg) == len(do) == cap
+    if cg["ties"] == 0 and co["ties"] == 0:
+        assert np.array_equal(dg, do)
+        assert ag == ao
+        assert np.array_equal(g.get_field(), o.get_field())
+        assert cg["uniforms_used"] == co["uniforms_used"]
+        assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+    # acceptance probabilities agree to rounding
+    assert np.abs(pg - po).max() < 1e-8 * max(1.0, np.abs(po).max())
+    assert g.state() == o.state() == (1, 1, 1)
+
+
+def test_u0_closed_form(pkg, hip):
+    """U = 0: G = [I + exp(-beta T)]^-1 at 1e-12 (flavortests_DQMC.jl:355-386)."""
+    for model in (pkg.HubbardModel(7, 2, U=0.0, t=1.0), pkg.HubbardModel(8, 2, U=0.0, mu=1.0, t=1.0)):
+        for beta in (1.0, 10.0):
+            par = pkg.DQMCParameters.create(beta=beta, delta_tau=0.1, safe_mult=5)
+            n = len(model.l)
+            g = hip.HIPChain(n, 1, par.slices, 5, 0.1, 0.0)
+            g.set_hopping(model.hopping_matrix())
+            g.set_field(np.ones((n, par.slices), dtype=np.int8, order="F"))
+            g.set_seed(1)
+            g.build_stack()
+            g.sweep(2)
+            w, V = np.linalg.eigh(model.hopping_matrix())
+            G = V @ np.diag(1.0 / (1.0 + np.exp(-beta * w))) @ V.T
+            Gd = g.greens(physical=True)[0]
+            assert np.abs(Gd - G).max() < 1e-12 + 1e-12 * np.abs(G).max()
+
+
+def test_config_b_sweep_parity(pkg, hip, oracle):
+    """Repulsive 16x16, beta=10 (N=256, M=100, 2 flavours): one full sweep against the oracle."""
+    g, o, model, par, field = pair(pkg, hip, oracle, 16, -4.0, 10.0, 0.0, seed=77)
+    n, M = 256, 100
+    u = np.random.default_rng(5).random(2 * n * M)
+    g.set_uniforms(u)
+    o.set_uniforms(u)
+    g.set_log(2 * n * M)
+    o.set_log(2 * n * M)
+    g.build_stack()
+    o.build_stack()
+    assert rel_maxdiff(g.greens(), o.greens()) < 1e-10
+    g.sweep(1)
+    o.sweep(1)
+    dg, pg = g.get_log()
+    do, po = o.get_log()
+    mism = int((dg != do).sum())
+    first = int(np.argmax(dg != do)) if mism else -1
+    cg, co = g.counters(), o.counters()
+    print("config B: mismatches", mism, "first", first, "ties", cg["ties"], co["ties"],
+          "acc", cg["accepted"], co["accepted"], "prop_err", cg["prop_err_last"], co["prop_err_last"])
+    if cg["ties"] == 0 and co["ties"] == 0:
+        assert mism == 0
+        assert np.array_equal(g.get_field(), o.get_field())
+        assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
