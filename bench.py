@@ -1,0 +1,210 @@
+#!/usr/bin/env python3
+"""DQMC sweep benchmark on one MI355X.
+
+Workload (one "step" = one sweep = one LocalSweep: 2M slice visits x N Metropolis proposals
+including every wrap and stabilisation, no measurements):
+  Repulsive Hubbard model, 16x16 periodic square lattice (N=256), t=1, U=4, mu=0, beta=10,
+  dtau=0.1 (M=100), magnetic Hirsch field (2 flavours, block diagonal), safe_mult=10,
+  one chain, seeded +-1 field, counter-stream uniforms.  FP64 throughout (headline) and the
+  mixed-precision form (FP32 wraps and rank-1 updates, FP64 UDT/recompute/detratio).
+
+Prints ONE JSON line.  Inputs are resident on the device before the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "DQMC sweeps/sec (N=256, M=100) + p50/p99 chain latency, mixed vs FP64"
+PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
+
+
+def load_pkg():
+    name = "montecarlo_jl_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    d = os.path.join(ROOT, "montecarlo.jl_amd")
+    spec = importlib.util.spec_from_file_location(name, os.path.join(d, "__init__.py"),
+                                                  submodule_search_locations=[d])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def config_b(pkg):
+    model = pkg.RepulsiveHubbardModel(16, 2, U=4.0)   # reference convention: U_ref = -4
+    par = pkg.DQMCParameters.create(beta=10.0, delta_tau=0.1, safe_mult=10)
+    field = model.choose_field()(par, model)
+    field.rand(1)
+    return model, par, field
+
+
+def make_chain(pkg, model, par, field, precision):
+    from montecarlo_jl_amd import _lib
+    ch = _lib.HIPChain(len(model.l), field.flavors, par.slices, par.safe_mult, par.delta_tau,
+                       field.alpha, check_propagation_error=True, precision=precision)
+    ch.set_hopping(model.hopping_matrix())
+    ch.set_field(field.conf)
+    ch.set_seed(12345)
+    ch.build_stack()
+    return ch
+
+
+def timed_sweeps(ch, steps, warmup):
+    for _ in range(warmup):
+        ch.sweep(1)
+    ch.synchronize()
+    lat = []
+    acc0 = ch.stats()["accepted"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        ch.sweep(1)            # returns after the sweep's device work is complete
+        lat.append(time.perf_counter() - a)
+    ch.synchronize()
+    total = time.perf_counter() - t0
+    acc = ch.stats()["accepted"] - acc0
+    lat = np.array(lat) * 1e3
+    return total, lat, acc
+
+
+def kernel_flops(kind, n, flv, accepted_per_launch=0.0):
+    """Algorithmic FLOP per launch of each kernel kind at block size n, flv blocks."""
+    if kind in ("gemm_f64", "gemm_f32"):
+        return 2.0 * flv * n ** 3                    # one n x n x n product per block
+    if kind == "udt":
+        return flv * (8.0 / 3.0) * n ** 3            # Householder R (4/3 n^3) + explicit Q (4/3 n^3)
+    if kind == "rdivp":
+        return flv * float(n) ** 3                   # triangular solve with n right-hand rows
+    if kind == "local":
+        return 2.0 * flv * n ** 2 * accepted_per_launch   # rank-1 updates of the accepted flips
+    return 0.0
+
+
+def cpu_baseline(pkg, model, par, field, sweeps):
+    """The CPU oracle (FP64 restatement of the reference algorithm), one core."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    O.build()
+    ch = O.OracleChain(len(model.l), field.flavors, par.slices, par.safe_mult, par.delta_tau,
+                       field.alpha)
+    ch.set_hopping(model.hopping_matrix())
+    ch.set_field(field.conf)
+    ch.set_seed(12345)
+    ch.build_stack()
+    t0 = time.perf_counter()
+    ch.sweep(sweeps)
+    dt = time.perf_counter() - t0
+    return sweeps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-sweeps", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: independent chains shard across GPUs with no "
+              "communication, so only --gpus 1 is run here", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    model, par, field = config_b(pkg)
+    n, flv, M = len(model.l), field.flavors, par.slices
+
+    # ---- FP64 headline ------------------------------------------------------------------
+    ch = make_chain(pkg, model, par, field, "fp64")
+    total, lat, acc = timed_sweeps(ch, args.steps, args.warmup)
+    sps = args.steps / total
+    st = ch.stats()
+
+    # ---- per-kernel device time (HIP events on the chain's stream) -------------------------
+    ksteps = max(2, args.steps // 4)
+    ch.set_kernel_timing(True)
+    acc_k0 = ch.stats()["accepted"]
+    t0 = time.perf_counter()
+    ch.sweep(ksteps)
+    t_events = time.perf_counter() - t0
+    kt = ch.kernel_times()
+    acc_k = ch.stats()["accepted"] - acc_k0
+    ch.set_kernel_timing(False)
+    kernels = {}
+    for kind, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        apl = acc_k / cnt if kind == "local" else 0.0
+        fl = kernel_flops(kind, n, flv, apl)
+        avg_s = ms / cnt / 1e3
+        kernels[kind] = {"launches_per_sweep": cnt / ksteps, "avg_us": avg_s * 1e6,
+                         "ms_per_sweep": ms / ksteps, "flop_per_launch": fl,
+                         "tflops": fl / avg_s / 1e12 if avg_s > 0 else 0.0}
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_sweep"])
+    peak = PEAK["fp32_mfma_tflops"] if dom == "gemm_f32" else PEAK["fp64_mfma_tflops"]
+    achieved = kernels[dom]["tflops"]
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "avg_launch_us": kernels[dom]["avg_us"],
+                "flop_per_launch": kernels[dom]["flop_per_launch"]}
+    ch.close()
+
+    # ---- mixed precision ---------------------------------------------------------------
+    mixed = None
+    if not args.skip_mixed:
+        cm = make_chain(pkg, model, par, field, "mixed")
+        cm.stabilization_log()
+        mt, mlat, macc = timed_sweeps(cm, args.steps, args.warmup)
+        dg = cm.stabilization_log()
+        mixed = {"value": args.steps / mt, "ms_per_step": mt / args.steps * 1e3,
+                 "p50_ms": float(np.percentile(mlat, 50)), "p99_ms": float(np.percentile(mlat, 99)),
+                 "max_dG_vs_fp64_recompute": float(dg.max()) if len(dg) else None,
+                 "stabilizations_checked": int(len(dg)),
+                 "speedup_vs_fp64": (args.steps / mt) / sps}
+        cm.close()
+
+    # ---- CPU baseline --------------------------------------------------------------------
+    cpu = None
+    if not args.skip_cpu:
+        cps, cdt = cpu_baseline(pkg, model, par, field, args.cpu_sweeps)
+        cpu = {"value": cps, "unit": "sweeps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_sweeps} sweeps of the same chain (N=256, M=100, 2 flavours) "
+                         f"after the stack build, {cdt:.1f} s, oracle/dqmc_oracle.c on one host core",
+               "host_cores_available": os.cpu_count(),
+               "gpu_over_cpu": sps / cps}
+
+    out = {
+        "metric": METRIC, "value": sps, "unit": "sweeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": total / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "repulsive Hubbard 16x16 square, N=256, t=1, U=4, mu=0, beta=10, "
+                               "dtau=0.1, M=100, magnetic Hirsch field (2 flavours), safe_mult=10, "
+                               "1 chain, FP64",
+                   "n_sites": n, "flavors": flv, "slices": M, "safe_mult": par.safe_mult},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "acceptance": acc / (args.steps * 2 * n * M),
+        "propagation_error_max": st["propagation_error_max"],
+        "propagation_error_last": st["propagation_error_last"],
+        "mixed": mixed,
+        "roofline": roofline,
+        "kernels": kernels,
+        "kernel_timing_region": {"sweeps": ksteps, "ms_per_sweep_with_events": t_events / ksteps * 1e3},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

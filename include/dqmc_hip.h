@@ -126,6 +126,18 @@ int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* stats);
 int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, int64_t* length);
 int dqmc_synchronize(dqmc_chain_t h);
 
+/* Kernel timing (the reference's @bm / TimerOutputs timers, helpers.jl:83-131): when
+ * enabled, every launch of the chain is bracketed by HIP events on the chain's stream and
+ * the device time is accumulated per kernel kind.  Enabling resets the totals.          */
+#define DQMC_KERNEL_GEMM_F64 0 /* slice products, wraps, calculate_greens products */
+#define DQMC_KERNEL_GEMM_F32 1 /* mixed-precision wraps                            */
+#define DQMC_KERNEL_UDT 2      /* pivoted QR / UDT                                 */
+#define DQMC_KERNEL_RDIVP 3    /* triangular solve of calculate_greens             */
+#define DQMC_KERNEL_LOCAL 4    /* Metropolis sweep over the sites of one slice     */
+#define DQMC_KERNEL_KINDS 5
+int dqmc_set_kernel_timing(dqmc_chain_t h, int enable);
+int dqmc_get_kernel_times(dqmc_chain_t h, double* total_ms, int64_t* launches, int kinds);
+
 /* Device linear algebra on host arrays (unit parity of the kernels).
  *   udt_AVX_pivot! (UDT.jl:216-334), rdivp! (real.jl:198-226),
  *   calculate_greens_AVX! (stack.jl:442-496), vmul! (real.jl:7-102).                   */

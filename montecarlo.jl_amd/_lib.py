@@ -45,8 +45,10 @@ EXPORTED = [
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_get_greens", "dqmc_set_greens",
     "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
-    "dqmc_la_calc_greens_udt", "dqmc_la_gemm",
+    "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
 ]
+
+KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local"]
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
@@ -92,6 +94,8 @@ def load():
         "dqmc_la_rdivp": [i, _dp, _dp, _i32],
         "dqmc_la_calc_greens_udt": [i, _dp, _dp, _dp, _dp, _dp, _dp, _dp],
         "dqmc_la_gemm": [i, i, i, _dp, _dp, _dp, i],
+        "dqmc_set_kernel_timing": [vp, i],
+        "dqmc_get_kernel_times": [vp, vp, vp, i],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -264,6 +268,18 @@ class HIPChain:
 
     def synchronize(self):
         self._c(load().dqmc_synchronize(self._h), "dqmc_synchronize")
+
+    def set_kernel_timing(self, enable: bool):
+        self._c(load().dqmc_set_kernel_timing(self._h, int(enable)), "dqmc_set_kernel_timing")
+
+    def kernel_times(self) -> dict:
+        """{kind: (total_ms, launches)} accumulated since timing was enabled."""
+        k = len(KERNEL_KINDS)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, dtype=np.int64)
+        self._c(load().dqmc_get_kernel_times(self._h, ms.ctypes.data, cnt.ctypes.data, k),
+                "dqmc_get_kernel_times")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(KERNEL_KINDS)}
 
 
 # ---- kernel-level entry points -----------------------------------------------------------

@@ -73,6 +73,14 @@ struct dqmc_chain_s {
   double perr_max = 0.0, perr_last = 0.0;
   std::vector<double> stablog;
   std::string err;
+
+  // optional per-kernel timing with HIP events on the chain's stream
+  bool prof = false;
+  struct Pending { int kind; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> evpool;
+  double kms[DQMC_KERNEL_KINDS] = {0};
+  long long kcount[DQMC_KERNEL_KINDS] = {0};
 };
 
 #define HIPCHK(h, x)                                                         \
@@ -151,8 +159,64 @@ GemmBatch<T> batch(int ta, int tb, double alpha = 1.0, double beta = 0.0) {
   return b;
 }
 
-int run(Chain* h, const GemmBatch<double>& b) { HIPCHK(h, launch_gemm_f64(b, h->st)); return DQMC_OK; }
-int run(Chain* h, const GemmBatch<float>& b) { HIPCHK(h, launch_gemm_f32(b, h->st)); return DQMC_OK; }
+hipEvent_t take_event(Chain* h) {
+  if (!h->evpool.empty()) {
+    hipEvent_t e = h->evpool.back();
+    h->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one launch with events when timing is on (kinds: DQMC_KERNEL_*).
+struct Timed {
+  Chain* h;
+  int kind;
+  hipEvent_t a = nullptr;
+  Timed(Chain* hh, int k) : h(hh), kind(k) {
+    if (h->prof) {
+      a = take_event(h);
+      if (a) (void)hipEventRecord(a, h->st);
+    }
+  }
+  ~Timed() {
+    if (a) {
+      hipEvent_t b = take_event(h);
+      if (b) {
+        (void)hipEventRecord(b, h->st);
+        h->pending.push_back({kind, a, b});
+      } else {
+        h->evpool.push_back(a);
+      }
+    }
+  }
+};
+
+void collect_times(Chain* h) {
+  for (auto& p : h->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      h->kms[p.kind] += ms;
+      h->kcount[p.kind] += 1;
+    }
+    h->evpool.push_back(p.a);
+    h->evpool.push_back(p.b);
+  }
+  h->pending.clear();
+}
+
+int run(Chain* h, const GemmBatch<double>& b) {
+  Timed t(h, DQMC_KERNEL_GEMM_F64);
+  HIPCHK(h, launch_gemm_f64(b, h->st));
+  return DQMC_OK;
+}
+int run(Chain* h, const GemmBatch<float>& b) {
+  Timed t(h, DQMC_KERNEL_GEMM_F32);
+  HIPCHK(h, launch_gemm_f32(b, h->st));
+  return DQMC_OK;
+}
 
 int copyd(Chain* h, double* dst, const double* src, size_t count) {
   HIPCHK(h, launch_copy(dst, src, (int64_t)count, h->st));
@@ -257,6 +321,7 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
     u.it[b].tfac = h->tfac + (size_t)b * (h->n + 16) * 16;
     u.it[b].swap = h->swp + (size_t)b * 32 * h->n;
   }
+  Timed t(h, DQMC_KERNEL_UDT);
   HIPCHK(h, launch_udt(u, h->st));
   return DQMC_OK;
 }
@@ -270,6 +335,7 @@ int rdivp(Chain* h, double* const A[], double* const T[]) {
     r.it[b].O = blk(h->O, h, b);
     r.it[b].pivot = h->piv + (size_t)b * h->n;
   }
+  Timed t(h, DQMC_KERNEL_RDIVP);
   HIPCHK(h, launch_rdivp(r, h->st));
   return DQMC_OK;
 }
@@ -403,6 +469,7 @@ int record_error(Chain* h, const double* a, const float* a32, const double* b) {
 
 int harvest(Chain* h) {
   HIPCHK(h, hipStreamSynchronize(h->st));
+  collect_times(h);
   if (h->nerr > 0) {
     std::vector<double> v(h->nerr);
     HIPCHK(h, hipMemcpy(v.data(), h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost));
@@ -517,6 +584,7 @@ int sweep_spatial(Chain* h) {
     a.kmax = h->kmax; a.lds_uv = h->lds_uv;
     a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
     a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
+    Timed t(h, DQMC_KERNEL_LOCAL);
     HIPCHK(h, launch_local_f32(a, h->st));
   } else {
     LocalArgs<double> a;
@@ -529,6 +597,7 @@ int sweep_spatial(Chain* h) {
     a.kmax = h->kmax; a.lds_uv = h->lds_uv;
     a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
     a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
+    Timed t(h, DQMC_KERNEL_LOCAL);
     HIPCHK(h, launch_local_f64(a, h->st));
   }
   return DQMC_OK;
@@ -711,6 +780,8 @@ int dqmc_destroy(dqmc_chain_t h) {
   if (!h) return DQMC_OK;
   (void)hipSetDevice(h->dev);
   if (h->st) (void)hipStreamSynchronize(h->st);
+  collect_times(h);
+  for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->unif) (void)hipFree(h->unif);
   if (h->dlog) (void)hipFree(h->dlog);
@@ -1042,6 +1113,24 @@ int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, in
   const int64_t k = std::min(len, capacity);
   if (out && k > 0) std::memcpy(out, h->stablog.data(), sizeof(double) * k);
   h->stablog.clear();
+  return DQMC_OK;
+}
+
+int dqmc_set_kernel_timing(dqmc_chain_t h, int enable) {
+  ENTER(h);
+  CHK(harvest(h));
+  h->prof = enable != 0;
+  for (int k = 0; k < DQMC_KERNEL_KINDS; k++) { h->kms[k] = 0.0; h->kcount[k] = 0; }
+  return DQMC_OK;
+}
+
+int dqmc_get_kernel_times(dqmc_chain_t h, double* total_ms, int64_t* launches, int kinds) {
+  ENTER(h);
+  CHK(harvest(h));
+  for (int k = 0; k < kinds && k < DQMC_KERNEL_KINDS; k++) {
+    if (total_ms) total_ms[k] = h->kms[k];
+    if (launches) launches[k] = h->kcount[k];
+  }
   return DQMC_OK;
 }
 
