@@ -133,8 +133,9 @@ int dqmc_synchronize(dqmc_chain_t h);
 #define DQMC_KERNEL_GEMM_F32 1 /* mixed-precision wraps                            */
 #define DQMC_KERNEL_UDT 2      /* pivoted QR / UDT                                 */
 #define DQMC_KERNEL_RDIVP 3    /* triangular solve of calculate_greens             */
-#define DQMC_KERNEL_LOCAL 4    /* Metropolis sweep over the sites of one slice     */
-#define DQMC_KERNEL_KINDS 5
+#define DQMC_KERNEL_LOCAL 4    /* Metropolis decisions of one window of sites      */
+#define DQMC_KERNEL_LOCAL_APPLY 5 /* rank-m application of a window's flips to G    */
+#define DQMC_KERNEL_KINDS 6
 int dqmc_set_kernel_timing(dqmc_chain_t h, int enable);
 int dqmc_get_kernel_times(dqmc_chain_t h, double* total_ms, int64_t* launches, int kinds);
 

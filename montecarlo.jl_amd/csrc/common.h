@@ -131,19 +131,19 @@ struct ChainCounters {
   long long flushes;
 };
 
+// Window kernel of the local update (local.hip): sites [w0, w0 + W) of one slice.
 template <typename GT>
-struct LocalArgs {
-  GT* G;                  // flv blocks, n x n, ld = n
+struct WindowArgs {
+  GT* G;                  // flv blocks, n x n, ld = n (read only here)
   int n, flv;
   int8_t* conf;           // conf column of the current slice
-  int slice;              // 1-based, for the log
+  int w0, W;
   double d_first[2];      // Delta block 1, index (x > 0)
   double d_second[2];     // Delta block 2 (magnetic)
   double bweight[2];      // exp(-dE_boson) density, index (x > 0)
-  GT* ubuf;               // flv * n * kmax (global spill for U columns)
-  GT* vbuf;               // flv * n * kmax
-  int kmax;
-  int lds_uv;             // U/V held in LDS
+  GT* C;                  // out: flv x (n x W)  (I - G)[:, S], zero-padded
+  GT* R;                  // out: flv x (W x n)  G[S, :], zero-padded
+  GT* Y;                  // out: flv x (W x W)
   const double* uniforms; // supplied stream (nullptr: counter stream)
   long long n_uniforms;
   unsigned long long seed;
@@ -152,8 +152,13 @@ struct LocalArgs {
   double* p_log;
   long long log_cap;
 };
-hipError_t launch_local_f64(const LocalArgs<double>& a, hipStream_t s);
-hipError_t launch_local_f32(const LocalArgs<float>& a, hipStream_t s);
-int local_lds_kmax(int n, int flv, int elem_bytes);
+hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
+hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
+int window_size(int n, int flv);
+// G_b -= C_b Y_b R_b for every block b (after a window kernel)
+hipError_t launch_window_apply_f64(double* G, const double* C, const double* R, const double* Y, int n,
+                                   int W, int flv, hipStream_t s);
+hipError_t launch_window_apply_f32(float* G, const float* C, const float* R, const float* Y, int n, int W,
+                                   int flv, hipStream_t s);
 
 }  // namespace dqmc

@@ -54,8 +54,8 @@ struct dqmc_chain_s {
   double *tau = nullptr, *tfac = nullptr, *swp = nullptr;
   int8_t* conf = nullptr;
   float *G32 = nullptr, *Gx32 = nullptr, *Gchk32 = nullptr, *eT2m32 = nullptr, *eT2p32 = nullptr;
-  void *ub = nullptr, *vb = nullptr;
-  int kmax = 16, lds_uv = 1;
+  double *wC = nullptr, *wR = nullptr, *wY = nullptr, *wZ = nullptr;  // window update factors
+  int win = 32;
   ChainCounters* ctr = nullptr;
   double* unif = nullptr;
   long long nunif = 0;
@@ -568,39 +568,46 @@ int forward_build(Chain* h) {
   return to32(h);
 }
 
+template <typename GT>
+int sweep_spatial_t(Chain* h, GT* G, GT* C, GT* R, GT* Y, GT* Z) {
+  const int n = h->n, W = h->win;
+  for (int w0 = 0; w0 < n; w0 += W) {
+    WindowArgs<GT> a;
+    a.G = G;
+    a.n = n; a.flv = h->flv;
+    a.conf = h->conf + (size_t)(h->cs - 1) * n;
+    a.w0 = w0; a.W = W;
+    for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
+    a.C = C; a.R = R; a.Y = Y;
+    a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
+    a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
+    {
+      Timed t(h, DQMC_KERNEL_LOCAL);
+      hipError_t e;
+      if constexpr (sizeof(GT) == 8) e = launch_window_f64(a, h->st);
+      else e = launch_window_f32(a, h->st);
+      HIPCHK(h, e);
+    }
+    {
+      Timed t(h, DQMC_KERNEL_LOCAL_APPLY);
+      hipError_t e;
+      if constexpr (sizeof(GT) == 8) e = launch_window_apply_f64(G, C, R, Y, n, W, h->flv, h->st);
+      else e = launch_window_apply_f32(G, C, R, Y, n, W, h->flv, h->st);
+      HIPCHK(h, e);
+    }
+  }
+  return DQMC_OK;
+}
+
 int sweep_spatial(Chain* h) {
   if (h->cs < 1 || h->cs > h->M) {
     h->err = "sweep_spatial: current slice outside 1..slices (build the stack first)";
     return DQMC_E_STATE;
   }
-  if (h->prec == DQMC_PRECISION_MIXED) {
-    LocalArgs<float> a;
-    a.G = h->G32;
-    a.n = h->n; a.flv = h->flv;
-    a.conf = h->conf + (size_t)(h->cs - 1) * h->n;
-    a.slice = h->cs;
-    for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
-    a.ubuf = reinterpret_cast<float*>(h->ub); a.vbuf = reinterpret_cast<float*>(h->vb);
-    a.kmax = h->kmax; a.lds_uv = h->lds_uv;
-    a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
-    a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
-    Timed t(h, DQMC_KERNEL_LOCAL);
-    HIPCHK(h, launch_local_f32(a, h->st));
-  } else {
-    LocalArgs<double> a;
-    a.G = h->G;
-    a.n = h->n; a.flv = h->flv;
-    a.conf = h->conf + (size_t)(h->cs - 1) * h->n;
-    a.slice = h->cs;
-    for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
-    a.ubuf = reinterpret_cast<double*>(h->ub); a.vbuf = reinterpret_cast<double*>(h->vb);
-    a.kmax = h->kmax; a.lds_uv = h->lds_uv;
-    a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
-    a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
-    Timed t(h, DQMC_KERNEL_LOCAL);
-    HIPCHK(h, launch_local_f64(a, h->st));
-  }
-  return DQMC_OK;
+  if (h->prec == DQMC_PRECISION_MIXED)
+    return sweep_spatial_t<float>(h, h->G32, reinterpret_cast<float*>(h->wC), reinterpret_cast<float*>(h->wR),
+                                  reinterpret_cast<float*>(h->wY), reinterpret_cast<float*>(h->wZ));
+  return sweep_spatial_t<double>(h, h->G, h->wC, h->wR, h->wY, h->wZ);
 }
 
 // exp(c T) on the device: Taylor series of degree 20 after scaling by 2^-s so that the
@@ -659,7 +666,9 @@ int64_t footprint(const dqmc_config_t* c, int* Kout) {
   b += f * n * (4 + 8 + 32 * 8) + f * (n + 16) * 16 * 8; // pivot, tau, swap, tfac
   b += n * (int64_t)c->slices;                       // conf
   if (c->precision == DQMC_PRECISION_MIXED) b += (3 * f * nn + 2 * nn) * 4;
-  b += 2 * f * n * 32 * 8;                           // delayed-update spill buffers
+  const int64_t W = window_size((int)n, (int)f);
+  const int64_t welem = c->precision == DQMC_PRECISION_MIXED ? 4 : 8;
+  b += (3 * f * n * W + f * W * W) * welem;          // window factors C, R, Z and Y
   b += 4096 * 8 + 4096;                              // error slots, counters
   return b;
 }
@@ -716,12 +725,8 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   h->bweight[0] = std::exp(-(-2.0 * a * -1.0));
   h->bweight[1] = std::exp(-(-2.0 * a * 1.0));
 
-  const int elem = h->prec == DQMC_PRECISION_MIXED ? 4 : 8;
-  int kl = local_lds_kmax(h->n, h->flv, elem);
-  int want = cfg->delay_depth > 0 ? (cfg->delay_depth + 3) / 4 * 4 : 32;
-  if (want > 64) want = 64;
-  if (kl >= 8) { h->lds_uv = 1; h->kmax = std::min(kl, want); }
-  else { h->lds_uv = 0; h->kmax = std::min(32, want); }
+  h->win = window_size(h->n, h->flv);
+  if (cfg->delay_depth > 0) h->win = std::max(8, std::min(h->win, (cfg->delay_depth + 7) / 8 * 8));
 
   const size_t f = h->flv, nn = h->nn, n = h->n;
   int r = DQMC_OK;
@@ -739,11 +744,8 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
     A(&h->G32, f * nn); A(&h->Gx32, f * nn); A(&h->Gchk32, f * nn);
     A(&h->eT2m32, nn); A(&h->eT2p32, nn);
   }
-  if (!h->lds_uv) {
-    double* p1 = nullptr; double* p2 = nullptr;
-    A(&p1, f * n * h->kmax); A(&p2, f * n * h->kmax);
-    h->ub = p1; h->vb = p2;
-  }
+  A(&h->wC, f * n * h->win); A(&h->wR, f * n * h->win); A(&h->wY, f * h->win * h->win);
+  A(&h->wZ, f * n * h->win);
   A(&h->ctr, 1);
   A(&h->errbuf, (size_t)h->errcap);
   A(&h->scal, 16);

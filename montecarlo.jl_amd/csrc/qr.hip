@@ -27,9 +27,26 @@ constexpr int kQrThreads = 256;
 constexpr int kQrMaxW = 16;
 constexpr int kPanelDoubles = 12 * 1024;  // 96 KB of panel storage
 
+// Wave64 sum of a double, result in every lane: butterflies inside each 16-lane row
+// with DPP (quad_perm xor 1, xor 2, row_ror 4, row_ror 8), then the four row sums are
+// combined through v_readlane.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
 __device__ __forceinline__ double wsum(double v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x124>(v);  // row_ror 4
+  v += dpp_mov<0x128>(v);  // row_ror 8
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
 static size_t udt_lds_bytes(int n) {
@@ -42,6 +59,95 @@ static size_t udt_lds_bytes(int n) {
   b += (size_t)(n + 1) * 4;           // panel starts
   b += 3 * 32 * 4 + 16;               // moves, selection, misc
   return b;
+}
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Entry (i, a) of the unit lower-trapezoidal reflector block V (panel-local row i),
+// zero outside the w reflectors of the panel.
+__device__ __forceinline__ double vget(const double* P, int rows, int w, int i, int a) {
+  if (i >= rows || a >= w) return 0.0;
+  return i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
+}
+
+// X[p0:n, c0:n] -= V op(T) V^T X[p0:n, c0:n] with op(T) = T^T (trans) or T, on the matrix
+// cores.  Each wave owns 16-column strips.  W = V^T X_strip and Y = op(T) W stay in
+// registers: the f64 16x16x4 accumulator element r of lane l is row l/16 + 4r, which is
+// exactly the B-operand element of k-chunk r, so W and Y feed the next products directly.
+// Loads are issued kChunk k-steps at a time so that many are in flight.  When nrm is
+// given, the squared norms of the updated columns over rows >= p0 + w are written to it
+// (the residual norms the next panel pivots on).
+constexpr int kChunk = 32;
+
+__device__ void apply_block_reflector(double* __restrict__ X, int n, int p0, int rows, int c0,
+                                      const double* P, int w, const double* Tf, bool trans,
+                                      int lane, int wave, double* nrm) {
+  constexpr int NW = kQrThreads / 64;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int cb = c0 + 16 * wave; cb < n; cb += 16 * NW) {
+    const int col = cb + li;
+    const bool colok = col < n;
+    const double* xc = X + (size_t)col * n + p0;
+    dx4 W = dx4{0, 0, 0, 0};
+    for (int kb = 0; kb < rows; kb += 4 * kChunk) {
+      double xb[kChunk];
+#pragma unroll
+      for (int q = 0; q < kChunk; q++) {
+        const int i = kb + 4 * q + lk;
+        xb[q] = (i < rows && colok) ? xc[i] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kChunk; q++) {
+        const int i = kb + 4 * q + lk;
+        if (kb + 4 * q < rows) W = mfma16(vget(P, rows, w, i, li), xb[q], W);
+      }
+    }
+    dx4 Y = dx4{0, 0, 0, 0};
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+      const int kk = 4 * kc + lk;
+      const double ta = trans ? Tf[kk * 16 + li] : Tf[li * 16 + kk];
+      Y = mfma16(ta, W[kc], Y);
+    }
+    double* xw = X + (size_t)col * n + p0;
+    double ss = 0.0;
+    constexpr int kRB = 4;  // 16-row blocks per batch
+    for (int rb = 0; rb < rows; rb += 16 * kRB) {
+      dx4 acc[kRB];
+#pragma unroll
+      for (int q = 0; q < kRB; q++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = rb + 16 * q + mfma_row<double>(lane, r);
+          acc[q][r] = (row < rows && colok) ? xw[row] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kRB; q++) {
+        const int r0 = rb + 16 * q;
+        if (r0 >= rows) break;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) acc[q] = mfma16(-vget(P, rows, w, r0 + li, 4 * kc + lk), Y[kc], acc[q]);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int row = r0 + mfma_row<double>(lane, r);
+          if (row < rows && colok) {
+            xw[row] = acc[q][r];
+            if (row >= w) ss += acc[q][r] * acc[q][r];
+          }
+        }
+      }
+    }
+    if (nrm) {
+      ss += __shfl_xor(ss, 16, 64);
+      ss += __shfl_xor(ss, 32, 64);
+      if (lk == 0 && colok) nrm[col] = ss;
+    }
+  }
 }
 
 __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
@@ -77,7 +183,9 @@ __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
     if (tid == 0) pst[npan] = p0;
     npan++;
 
-    // 1. exact residual norms of the trailing columns
+    // 1. exact residual norms of the trailing columns (later panels: written by the
+    //    previous panel's trailing update)
+    if (p0 == 0)
     for (int k = p0 + wave; k < n; k += NW) {
       double s = 0.0;
       for (int i = p0 + lane; i < n; i += 64) {
@@ -94,7 +202,8 @@ __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
     for (int k = p0 + tid; k < n; k += kQrThreads) {
       const double v = nrm[k];
       int cnt = 0;
-      for (int k2 = p0; k2 < n && cnt < w; k2++) {
+#pragma unroll 8
+      for (int k2 = p0; k2 < n; k2++) {
         const double u = nrm[k2];
         cnt += (u > v) || (u == v && k2 < k);
       }
@@ -138,14 +247,16 @@ __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
 
     // 4. physical column moves through the scratch buffer
     const int nm = misc[0];
-    for (int idx = tid; idx < nm * n; idx += kQrThreads) {
-      const int q = idx / n, r = idx - q * n;
-      it.swap[(size_t)q * n + r] = A[(size_t)r + (size_t)mv_src[q] * n];
+    for (int q = wave; q < nm; q += NW) {
+      const double* src = A + (size_t)mv_src[q] * n;
+      double* dst = it.swap + (size_t)q * n;
+      for (int r = lane; r < n; r += 64) dst[r] = src[r];
     }
     __syncthreads();
-    for (int idx = tid; idx < nm * n; idx += kQrThreads) {
-      const int q = idx / n, r = idx - q * n;
-      A[(size_t)r + (size_t)mv_dst[q] * n] = it.swap[(size_t)q * n + r];
+    for (int q = wave; q < nm; q += NW) {
+      const double* src = it.swap + (size_t)q * n;
+      double* dst = A + (size_t)mv_dst[q] * n;
+      for (int r = lane; r < n; r += 64) dst[r] = src[r];
     }
     __syncthreads();
 
@@ -196,75 +307,49 @@ __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
       A[(size_t)(p0 + i) + (size_t)(p0 + jj) * n] = P[idx];
     }
 
-    // 6. compact-WY factor T (upper triangular, w x w): H_1..H_w = I - V T V^T
-    const int npairs = w * (w - 1) / 2;
-    for (int pr = wave; pr < npairs; pr += NW) {
-      int a = 0, rem = pr;
-      while (rem >= w - 1 - a) { rem -= w - 1 - a; a++; }
-      const int b = a + 1 + rem;
-      double s = 0.0;
-      for (int i = b + 1 + lane; i < rows; i += 64) s += P[i + (size_t)rows * a] * P[i + (size_t)rows * b];
-      s = wsum(s);
-      if (lane == 0) S[a * 16 + b] = s + P[b + (size_t)rows * a];
-    }
+    // 6. compact-WY factor T (upper triangular, 16 x 16, zero beyond w):
+    //    H_1..H_w = I - V T V^T.  S = V^T V on the matrix cores (wave 0), then lane i of
+    //    wave 0 builds row i of T from its own earlier entries:
+    //    T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j].
     __syncthreads();
-    if (tid == 0) {
-      for (int j = 0; j < w; j++) {
-        const double tj = sc[j];
-        for (int i = 0; i < j; i++) {
-          double s = 0.0;
-          for (int k = i; k < j; k++) s += Tf[i * 16 + k] * S[k * 16 + j];
-          Tf[i * 16 + j] = -tj * s;
-        }
-        Tf[j * 16 + j] = tj;
-        for (int i = j + 1; i < 16; i++) Tf[i * 16 + j] = 0.0;
+    if (wave == 0) {
+      dx4 Sacc = dx4{0, 0, 0, 0};
+      for (int k0 = 0; k0 < rows; k0 += 4) {
+        const double v = vget(P, rows, w, k0 + (lane >> 4), lane & 15);
+        Sacc = mfma16(v, v, Sacc);
       }
-      for (int j = w; j < 16; j++)
-        for (int i = 0; i < 16; i++) Tf[i * 16 + j] = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; r++) S[mfma_row<double>(lane, r) * 16 + (lane & 15)] = Sacc[r];
     }
     __syncthreads();
-    for (int idx = tid; idx < 256; idx += kQrThreads) it.tfac[(size_t)p0 * 16 + idx] = Tf[idx];
+    if (tid < 16) {
+      const int i = tid;
+      double row[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        double t = 0.0;
+        if (i < w && j < w) {
+          if (j == i) t = sc[j];
+          else if (j > i) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 16; k++)
+              if (k >= i && k < j) s += row[k] * S[k * 16 + j];
+            t = -sc[j] * s;
+          }
+        }
+        row[j] = t;
+      }
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        Tf[i * 16 + j] = row[j];
+        it.tfac[(size_t)p0 * 16 + i * 16 + j] = row[j];
+      }
+    }
+    __syncthreads();
 
     // 7. trailing update A <- (I - V T^T V^T) A for the columns right of the panel
-    for (int k = p0 + w + wave; k < n; k += NW) {
-      double y[kQrMaxW];
-#pragma unroll
-      for (int a = 0; a < kQrMaxW; a++) y[a] = 0.0;
-      double* col = A + (size_t)k * n + p0;
-      for (int i = lane; i < rows; i += 64) {
-        const double x = col[i];
-#pragma unroll
-        for (int a = 0; a < kQrMaxW; a++) {
-          if (a >= w) break;
-          const double v = i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
-          y[a] += v * x;
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < kQrMaxW; a++) {
-        if (a >= w) break;
-        y[a] = wsum(y[a]);
-      }
-      double z[kQrMaxW];
-#pragma unroll
-      for (int b = 0; b < kQrMaxW; b++) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < kQrMaxW; a++)
-          if (a <= b) s += Tf[a * 16 + b] * y[a];
-        z[b] = s;
-      }
-      for (int i = lane; i < rows; i += 64) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < kQrMaxW; a++) {
-          if (a >= w) break;
-          const double v = i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
-          s += v * z[a];
-        }
-        col[i] -= s;
-      }
-    }
+    apply_block_reflector(A, n, p0, rows, p0 + w, P, w, Tf, true, lane, wave, nrm);
     __syncthreads();
     p0 += w;
   }
@@ -286,45 +371,7 @@ __global__ __launch_bounds__(kQrThreads) void udt_kernel(UdtBatch batch) {
     }
     for (int idx = tid; idx < 256; idx += kQrThreads) Tf[idx] = it.tfac[(size_t)p0 * 16 + idx];
     __syncthreads();
-    for (int k = p0 + wave; k < n; k += NW) {
-      double y[kQrMaxW];
-#pragma unroll
-      for (int a = 0; a < kQrMaxW; a++) y[a] = 0.0;
-      double* col = U + (size_t)k * n + p0;
-      for (int i = lane; i < rows; i += 64) {
-        const double x = col[i];
-#pragma unroll
-        for (int a = 0; a < kQrMaxW; a++) {
-          if (a >= w) break;
-          const double v = i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
-          y[a] += v * x;
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < kQrMaxW; a++) {
-        if (a >= w) break;
-        y[a] = wsum(y[a]);
-      }
-      double z[kQrMaxW];
-#pragma unroll
-      for (int a = 0; a < kQrMaxW; a++) {
-        double s = 0.0;
-#pragma unroll
-        for (int b = 0; b < kQrMaxW; b++)
-          if (b >= a && b < w) s += Tf[a * 16 + b] * y[b];
-        z[a] = s;
-      }
-      for (int i = lane; i < rows; i += 64) {
-        double s = 0.0;
-#pragma unroll
-        for (int a = 0; a < kQrMaxW; a++) {
-          if (a >= w) break;
-          const double v = i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
-          s += v * z[a];
-        }
-        col[i] -= s;
-      }
-    }
+    apply_block_reflector(U, n, p0, rows, p0, P, w, Tf, false, lane, wave, nullptr);
     __syncthreads();
   }
 
@@ -371,47 +418,75 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
   return hipGetLastError();
 }
 
-// rdivp: A <- A[:, pivot] * triu(T)^-1.  Rows are independent: one lane per row,
-// forward substitution over columns in blocks of 16 (T entries are wave-uniform).
-__global__ __launch_bounds__(64) void rdivp_kernel(RdivpBatch batch) {
+// rdivp: A <- A[:, pivot] * triu(T)^-1 (only the upper triangle of T is read).
+// Rows are independent.  Each wave owns 16 rows; for every 16-column panel J the
+// coupling to the already solved columns, X_{<J} T_{<J,J}, is a 16 x 16 x j0 product on
+// the matrix cores, then the 16 x 16 triangular block is solved by 16 lanes (one row each)
+// in the reference's order: x_j = (o_j - sum_{k<j} x_k T_kj) / T_jj.
+__global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
   const RdivpItem it = batch.it[blockIdx.y];
   const int n = batch.n;
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= n) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * 64 + wave * 16;
+  __shared__ double Ysh[4][16][17];
+  __shared__ double Tsh[4][16][17];
+  if (r0 >= n) return;  // per-wave work only: no workgroup barrier below
   double* __restrict__ A = it.A;
   double* __restrict__ O = it.O;
   const double* __restrict__ T = it.T;
-  for (int j = 0; j < n; j++) O[(size_t)i + (size_t)j * n] = A[(size_t)i + (size_t)it.pivot[j] * n];
+  const int* __restrict__ piv = it.pivot;
+  {
+    const int r = r0 + li;
+    if (r < n)
+      for (int j = lk; j < n; j += 4) O[(size_t)r + (size_t)j * n] = A[(size_t)r + (size_t)piv[j] * n];
+  }
+  __threadfence_block();
   for (int j0 = 0; j0 < n; j0 += 16) {
     const int w = n - j0 < 16 ? n - j0 : 16;
-    double acc[16];
+    dx4 acc = dx4{0, 0, 0, 0};
+    const int ra = r0 + li;
+    const int cb = j0 + li;
+#pragma unroll 4
+    for (int k0 = 0; k0 < j0; k0 += 4) {
+      const int k = k0 + lk;
+      const double av = ra < n ? A[(size_t)ra + (size_t)k * n] : 0.0;
+      const double bv = cb < n ? T[(size_t)k + (size_t)cb * n] : 0.0;
+      acc = mfma16(av, bv, acc);
+    }
 #pragma unroll
-    for (int c = 0; c < 16; c++) acc[c] = c < w ? O[(size_t)i + (size_t)(j0 + c) * n] : 0.0;
-    for (int k = 0; k < j0; k++) {
-      const double x = A[(size_t)i + (size_t)k * n];
+    for (int r = 0; r < 4; r++) {
+      const int row = mfma_row<double>(lane, r);
+      const int gr = r0 + row, gc = j0 + li;
+      Ysh[wave][row][li] = (gr < n && gc < n) ? O[(size_t)gr + (size_t)gc * n] - acc[r] : 0.0;
+      const int tr = j0 + row;
+      Tsh[wave][row][li] = (tr < n && gc < n) ? T[(size_t)tr + (size_t)gc * n] : 1.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_s_waitcnt(0);
+    if (lane < 16 && r0 + lane < n) {
+      double x[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        double v = Ysh[wave][lane][c];
+#pragma unroll
+        for (int cp = 0; cp < 16; cp++)
+          if (cp < c) v -= x[cp] * Tsh[wave][cp][c];
+        x[c] = v / Tsh[wave][c][c];
+      }
 #pragma unroll
       for (int c = 0; c < 16; c++)
-        if (c < w) acc[c] -= x * T[(size_t)k + (size_t)(j0 + c) * n];
+        if (c < w) A[(size_t)(r0 + lane) + (size_t)(j0 + c) * n] = x[c];
     }
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-      if (c >= w) break;
-      double x = acc[c];
-#pragma unroll
-      for (int cp = 0; cp < 16; cp++)
-        if (cp < c) x -= acc[cp] * T[(size_t)(j0 + cp) + (size_t)(j0 + c) * n];
-      acc[c] = x / T[(size_t)(j0 + c) + (size_t)(j0 + c) * n];
-    }
-#pragma unroll
-    for (int c = 0; c < 16; c++)
-      if (c < w) A[(size_t)i + (size_t)(j0 + c) * n] = acc[c];
+    __threadfence_block();
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
 hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
   if (b.count <= 0) return hipSuccess;
   if (b.count > kMaxUdtItems) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rdivp_kernel, dim3((b.n + 63) / 64, b.count), dim3(64), 0, s, b);
+  hipLaunchKernelGGL(rdivp_kernel, dim3((b.n + 63) / 64, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 

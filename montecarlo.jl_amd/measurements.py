@@ -1,0 +1,79 @@
+"""Equal-time measurements from the physical Green's function (host side).
+
+Mirrors the reference's Wick kernels for the Hubbard model:
+  occupation      measurements/constructors/occupation.jl:44-55   <n_i> = 1 - G_ii per flavour
+  kinetic energy  measurements/constructors/energy.jl:119-150     sum_ij T_ij (I - G)_ij
+                  (x 2 for the spin-degenerate density field)
+  interaction     models/HubbardModel.jl:160-182                   -U sum_i (G_up_ii - 1/2)(G_dn_ii - 1/2)
+  total energy    measurements/constructors/energy.jl:162-166
+Double occupancy <n_up n_dn> = (1 - G_up_ii)(1 - G_dn_ii) for one auxiliary-field
+configuration (the two spin sectors are independent at fixed field); it is not a
+reference measurement and is checked against exact diagonalisation.
+
+`G` is the physical Green's function as (flavors, N, N): one block for the density
+field (both spins identical), two blocks (up, down) for the magnetic field.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def spin_blocks(G: np.ndarray):
+    return (G[0], G[0]) if G.shape[0] == 1 else (G[0], G[1])
+
+
+def occupation(G: np.ndarray) -> np.ndarray:
+    """Per site and flavour block: 1 - G_ii (flavour-major)."""
+    return np.concatenate([1.0 - np.diag(g) for g in G])
+
+
+def density(G: np.ndarray) -> float:
+    """<n> per site, both spins."""
+    up, dn = spin_blocks(G)
+    return float(np.mean(2.0 - np.diag(up) - np.diag(dn)))
+
+
+def double_occupancy(G: np.ndarray) -> np.ndarray:
+    up, dn = spin_blocks(G)
+    return (1.0 - np.diag(up)) * (1.0 - np.diag(dn))
+
+
+def kinetic_energy(G: np.ndarray, T: np.ndarray) -> float:
+    I = np.eye(T.shape[0])
+    e = sum(float(np.sum(T * (I - g))) for g in G)
+    return 2.0 * e if G.shape[0] == 1 else e
+
+
+def interaction_energy(G: np.ndarray, U: float) -> float:
+    up, dn = spin_blocks(G)
+    return float(-U * np.sum((np.diag(up) - 0.5) * (np.diag(dn) - 0.5)))
+
+
+def total_energy(G: np.ndarray, T: np.ndarray, U: float) -> float:
+    return kinetic_energy(G, T) + interaction_energy(G, U)
+
+
+class MeasurementSet:
+    """Running means of the equal-time observables (one sample per call)."""
+
+    def __init__(self, T: np.ndarray, U: float):
+        self.T, self.U = T, U
+        self.count = 0
+        self.sums = {}
+
+    def push(self, G: np.ndarray):
+        vals = {
+            "greens": G.copy(),
+            "occupation": occupation(G),
+            "double_occupancy": double_occupancy(G),
+            "density": density(G),
+            "kinetic_energy": kinetic_energy(G, self.T),
+            "interaction_energy": interaction_energy(G, self.U),
+        }
+        vals["total_energy"] = vals["kinetic_energy"] + vals["interaction_energy"]
+        for k, v in vals.items():
+            self.sums[k] = self.sums.get(k, 0.0) + v
+        self.count += 1
+
+    def mean(self, key: str):
+        return self.sums[key] / max(self.count, 1)
