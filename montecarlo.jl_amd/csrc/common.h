@@ -134,16 +134,15 @@ struct ChainCounters {
 // Window kernel of the local update (local.hip): sites [w0, w0 + W) of one slice.
 template <typename GT>
 struct WindowArgs {
-  GT* G;                  // flv blocks, n x n, ld = n (read only here)
+  const GT* G;            // flv blocks, n x n, ld = n (read only here)
   int n, flv;
   int8_t* conf;           // conf column of the current slice
-  int w0, W;
+  int w0, W;              // W <= 64
   double d_first[2];      // Delta block 1, index (x > 0)
   double d_second[2];     // Delta block 2 (magnetic)
   double bweight[2];      // exp(-dE_boson) density, index (x > 0)
-  GT* C;                  // out: flv x (n x W)  (I - G)[:, S], zero-padded
-  GT* R;                  // out: flv x (W x n)  G[S, :], zero-padded
-  GT* Y;                  // out: flv x (W x W)
+  int* S;                 // out: S[0] = m accepted, S[1 + k] = site of flip k (0-based)
+  GT* Y;                  // out: flv x (W x W), zero beyond m
   const double* uniforms; // supplied stream (nullptr: counter stream)
   long long n_uniforms;
   unsigned long long seed;
@@ -155,10 +154,11 @@ struct WindowArgs {
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
 int window_size(int n, int flv);
-// G_b -= C_b Y_b R_b for every block b (after a window kernel)
-hipError_t launch_window_apply_f64(double* G, const double* C, const double* R, const double* Y, int n,
-                                   int W, int flv, hipStream_t s);
-hipError_t launch_window_apply_f32(float* G, const float* C, const float* R, const float* Y, int n, int W,
+size_t window_lds_bytes(int W, int flv);
+// Gd_b = Gs_b - C_b Y_b R_b for every block b, C = (I - Gs)[:, S], R = Gs[S, :]
+hipError_t launch_window_apply_f64(double* Gd, const double* Gs, const int* S, const double* Y, int n, int W,
+                                   int flv, hipStream_t s);
+hipError_t launch_window_apply_f32(float* Gd, const float* Gs, const int* S, const float* Y, int n, int W,
                                    int flv, hipStream_t s);
 
 }  // namespace dqmc

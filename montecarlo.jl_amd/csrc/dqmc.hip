@@ -54,7 +54,10 @@ struct dqmc_chain_s {
   double *tau = nullptr, *tfac = nullptr, *swp = nullptr;
   int8_t* conf = nullptr;
   float *G32 = nullptr, *Gx32 = nullptr, *Gchk32 = nullptr, *eT2m32 = nullptr, *eT2p32 = nullptr;
-  double *wC = nullptr, *wR = nullptr, *wY = nullptr, *wZ = nullptr;  // window update factors
+  double* Gpp = nullptr;   // second copy of G for the window updates (FP64 form)
+  float* G32pp = nullptr;  // second copy of G32 (mixed form)
+  double* wY = nullptr;    // window factor Y (flv x W x W, FP64 or FP32 contents)
+  int* wS = nullptr;       // window flips: count and sites
   int win = 32;
   ChainCounters* ctr = nullptr;
   double* unif = nullptr;
@@ -568,17 +571,21 @@ int forward_build(Chain* h) {
   return to32(h);
 }
 
+// The windows alternate between the two copies of G (each window reads one and writes
+// the other); after an odd number of windows the copies swap roles.
 template <typename GT>
-int sweep_spatial_t(Chain* h, GT* G, GT* C, GT* R, GT* Y, GT* Z) {
+int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
   const int n = h->n, W = h->win;
+  GT* src = *Gp;
+  GT* dst = *Gpp;
   for (int w0 = 0; w0 < n; w0 += W) {
     WindowArgs<GT> a;
-    a.G = G;
+    a.G = src;
     a.n = n; a.flv = h->flv;
     a.conf = h->conf + (size_t)(h->cs - 1) * n;
     a.w0 = w0; a.W = W;
     for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
-    a.C = C; a.R = R; a.Y = Y;
+    a.S = h->wS; a.Y = Y;
     a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
     a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
     {
@@ -591,11 +598,14 @@ int sweep_spatial_t(Chain* h, GT* G, GT* C, GT* R, GT* Y, GT* Z) {
     {
       Timed t(h, DQMC_KERNEL_LOCAL_APPLY);
       hipError_t e;
-      if constexpr (sizeof(GT) == 8) e = launch_window_apply_f64(G, C, R, Y, n, W, h->flv, h->st);
-      else e = launch_window_apply_f32(G, C, R, Y, n, W, h->flv, h->st);
+      if constexpr (sizeof(GT) == 8) e = launch_window_apply_f64(dst, src, h->wS, Y, n, W, h->flv, h->st);
+      else e = launch_window_apply_f32(dst, src, h->wS, Y, n, W, h->flv, h->st);
       HIPCHK(h, e);
     }
+    std::swap(src, dst);
   }
+  *Gp = src;
+  *Gpp = dst;
   return DQMC_OK;
 }
 
@@ -605,9 +615,8 @@ int sweep_spatial(Chain* h) {
     return DQMC_E_STATE;
   }
   if (h->prec == DQMC_PRECISION_MIXED)
-    return sweep_spatial_t<float>(h, h->G32, reinterpret_cast<float*>(h->wC), reinterpret_cast<float*>(h->wR),
-                                  reinterpret_cast<float*>(h->wY), reinterpret_cast<float*>(h->wZ));
-  return sweep_spatial_t<double>(h, h->G, h->wC, h->wR, h->wY, h->wZ);
+    return sweep_spatial_t<float>(h, &h->G32, &h->G32pp, reinterpret_cast<float*>(h->wY));
+  return sweep_spatial_t<double>(h, &h->G, &h->Gpp, h->wY);
 }
 
 // exp(c T) on the device: Taylor series of degree 20 after scaling by 2^-s so that the
@@ -668,7 +677,7 @@ int64_t footprint(const dqmc_config_t* c, int* Kout) {
   if (c->precision == DQMC_PRECISION_MIXED) b += (3 * f * nn + 2 * nn) * 4;
   const int64_t W = window_size((int)n, (int)f);
   const int64_t welem = c->precision == DQMC_PRECISION_MIXED ? 4 : 8;
-  b += (3 * f * n * W + f * W * W) * welem;          // window factors C, R, Z and Y
+  b += (f * nn + f * W * W) * welem + (W + 1) * 4;   // second copy of G, window factors Y, S
   b += 4096 * 8 + 4096;                              // error slots, counters
   return b;
 }
@@ -727,6 +736,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
 
   h->win = window_size(h->n, h->flv);
   if (cfg->delay_depth > 0) h->win = std::max(8, std::min(h->win, (cfg->delay_depth + 7) / 8 * 8));
+  if (h->win > 64) h->win = 64;
 
   const size_t f = h->flv, nn = h->nn, n = h->n;
   int r = DQMC_OK;
@@ -741,11 +751,12 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   A(&h->piv, f * n); A(&h->tau, f * n); A(&h->tfac, f * (n + 16) * 16); A(&h->swp, f * 32 * n);
   A(&h->conf, n * (size_t)h->M);
   if (h->prec == DQMC_PRECISION_MIXED) {
-    A(&h->G32, f * nn); A(&h->Gx32, f * nn); A(&h->Gchk32, f * nn);
+    A(&h->G32, f * nn); A(&h->Gx32, f * nn); A(&h->Gchk32, f * nn); A(&h->G32pp, f * nn);
     A(&h->eT2m32, nn); A(&h->eT2p32, nn);
+  } else {
+    A(&h->Gpp, f * nn);
   }
-  A(&h->wC, f * n * h->win); A(&h->wR, f * n * h->win); A(&h->wY, f * h->win * h->win);
-  A(&h->wZ, f * n * h->win);
+  A(&h->wY, f * h->win * h->win); A(&h->wS, (size_t)h->win + 1);
   A(&h->ctr, 1);
   A(&h->errbuf, (size_t)h->errcap);
   A(&h->scal, 16);

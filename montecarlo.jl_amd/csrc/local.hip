@@ -5,15 +5,12 @@
 // accept_local! / update_greens! (fields.jl:271-286, 340-344) with the rank-1 helpers
 // of linalg/updates.jl:7-131.
 //
-// Design (one workgroup per chain, all flavour blocks):
-//  * Decisions are evaluated redundantly by every thread from the running diagonal
-//    d_p = G'_pp kept in LDS, so a rejected proposal costs no barrier.
-//  * Accepted flips are not applied to G immediately ("delayed update"): with m pending
-//    flips G' = G - U V^T, where column j of U is (e_i - G'[:, i]) and column j of V is
-//    (Delta/R) G'[i, :]^T of the j-th accepted flip.  The column/row of G' needed by a
-//    new flip cost O(n m); the diagonal is updated in O(n).
-//  * After kmax accepted flips (or at the end of the slice) G <- G - U V^T is applied
-//    as a rank-kmax product on the matrix cores (16x16x4 MFMA tiles).
+// Design: the sites of a slice are visited in windows of W <= 64 consecutive sites.
+//  * window_kernel (one workgroup per chain): Metropolis decisions and rank-1 updates on
+//    the W x W block of G held in registers, and the factors S, Y of the accumulated
+//    update G <- G - C Y R with C = (I - G)[:, S], R = G[S, :].
+//  * window_apply_kernel (one 64 x 64 tile per workgroup): the rank-m update of all of G
+//    on the matrix cores, from one copy of G into the other.
 // This is algebraically the reference's sequence of rank-1 updates; only the rounding
 // differs, so accept decisions agree except at ties |u - p| ~ 1e-12, which are counted.
 #include "common.h"
@@ -74,74 +71,78 @@ __device__ __forceinline__ double gamma_of(double D, double R) {
   return D / R;
 }
 
+__device__ __forceinline__ double readlane_f64(double x, int l) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)v, l);
+  const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // ---------------------------------------------------------------------------------------
-// Windowed form: the decisions of W consecutive sites need only the W x W block of G.
+// Windowed form: the decisions of W <= 64 consecutive sites need only the W x W block of G.
 // A window kernel (one workgroup per chain) runs the Metropolis decisions and the rank-1
-// updates on that block in LDS, and accumulates, by bordering, the m x m factors of
+// updates on that block, and forms the factors of
 //     G_new = G - C Y R,   C = (I - G)[:, S],  R = G[S, :],
 //     Y = (I - N)^-1 (I + Gamma L)^-1 Gamma,
-// with S the accepted sites, N[l][j] = v_l(s_j), L[j][l] = u_l(s_j) (l < j), u_l and v_l
-// the rank-1 vectors of flip l.  In exact arithmetic this is the reference's sequence of
-// rank-1 updates.  The O(n^2 m) application is two MFMA GEMM launches over many CUs.
+// with S the accepted sites, u_l = e_{s_l} - G_l[:, s_l], v_l = gamma_l G_l[s_l, :] the
+// rank-1 vectors of flip l (G_l: after l flips), L[j][l] = u_l(s_j) and N[l][j] = v_l(s_j)
+// (l < j).  In exact arithmetic this is the reference's sequence of rank-1 updates.
+//
+// Register layout: lane = row of the window block; wave w keeps columns w, w+4, ... of
+// every flavour block in registers, and every wave keeps its own copy of the diagonal
+// (lane p holds G_pp), so a decision needs no memory access and a rejected proposal no
+// barrier.  An accepted flip at site t: the owner of column t publishes u, lane t of
+// every wave publishes v on its columns (LDS), one barrier, then every wave applies the
+// rank-1 update to its columns and to its diagonal copy with the same fma, so the copies
+// stay bitwise equal to the block.  The published u, v columns are kept for the solves.
 // ---------------------------------------------------------------------------------------
-template <typename GT, int NT>
-__global__ __launch_bounds__(NT) void window_kernel(WindowArgs<GT> a) {
+template <typename GT>
+__global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
+  constexpr int NW = 4, kCPW = 16, kQ = 32;
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = a.n, flv = a.flv, W = a.W, w0 = a.w0;
   const int Wn = n - w0 < W ? n - w0 : W;
   const int LD = W + 1;
-  const int tid = threadIdx.x;
-  double* Gw = reinterpret_cast<double*>(smem);           // [b][row][col], flv*W*LD
-  double* Uw = Gw + (size_t)flv * W * LD;                 // [b][site][flip]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // LDS, per flavour block b: Uw[b][l][r] = u_l(r), Vw[b][l][c] = v_l(c)
+  double* Uw = reinterpret_cast<double*>(smem);
   double* Vw = Uw + (size_t)flv * W * LD;
-  double* Kf = Vw + (size_t)flv * W * LD;                 // [b][l][l'] lower triangular
-  double* Pf = Kf + (size_t)flv * W * LD;                 // [b][l][l'] upper triangular
-  double* tu = Pf + (size_t)flv * W * LD;                 // [b][W] new u
-  double* tv = tu + (size_t)flv * W;                      // [b][W] new v
-  double* tk = tv + (size_t)flv * W;                      // [b][W] new row of K
-  double* tp = tk + (size_t)flv * W;                      // [b][W] new column of Pinv
-  double* us = tp + (size_t)flv * W;                      // staged uniforms [W]
-  int* sites = reinterpret_cast<int*>(us + W);            // [W]
-  int8_t* cf = reinterpret_cast<int8_t*>(sites + W);      // [W]
-  GT* __restrict__ G = a.G;
+  const GT* __restrict__ G = a.G;
+  const size_t nn = (size_t)n * n;
 
   const long long pos0 = a.ctr->u_pos;
-  for (int idx = tid; idx < flv * W * LD; idx += NT) {
-    Kf[idx] = 0.0;
-    Pf[idx] = 0.0;
+  long long loglen = a.ctr->log_len;
+  const bool rowok = lane < Wn;
+  double g0[kCPW], g1[kCPW];
+#pragma unroll
+  for (int i = 0; i < kCPW; i++) {
+    const int c = wave + NW * i;
+    const bool ok = rowok && c < Wn;
+    const size_t o = (size_t)(w0 + lane) + (size_t)(w0 + c) * n;
+    g0[i] = ok ? (double)G[o] : 0.0;
+    g1[i] = (ok && flv > 1) ? (double)G[nn + o] : 0.0;
   }
-  // window block of G, rows fastest (coalesced), several loads in flight per thread
-  {
-    const int total = flv * W * W;
-#pragma unroll 4
-    for (int idx = tid; idx < total; idx += NT) {
-      const int b = idx / (W * W), rem = idx - b * W * W, c = rem / W, r = rem - c * W;
-      Gw[(size_t)b * W * LD + (size_t)r * LD + c] =
-          (r < Wn && c < Wn) ? (double)G[(size_t)b * n * n + (size_t)(w0 + r) + (size_t)(w0 + c) * n] : 0.0;
-    }
-  }
-  for (int t = tid; t < Wn; t += NT) {
-    cf[t] = a.conf[w0 + t];
-    if (a.uniforms) {
-      const long long k = pos0 + t;
-      us[t] = k < a.n_uniforms ? a.uniforms[k] : 1.0;
-    }
-  }
-  __syncthreads();
+  const size_t od = (size_t)(w0 + lane) * (n + 1);
+  double d0 = rowok ? (double)G[od] : 0.0;
+  double d1 = (rowok && flv > 1) ? (double)G[nn + od] : 0.0;
+  const int cfl = rowok ? (int)a.conf[w0 + lane] : 0;
+  double ul = 1.0;
+  if (a.uniforms && pos0 + lane < a.n_uniforms) ul = a.uniforms[pos0 + lane];
 
   const double df0 = a.d_first[0], df1 = a.d_first[1];
   const double ds0 = a.d_second[0], ds1 = a.d_second[1];
   const double bw0 = a.bweight[0], bw1 = a.bweight[1];
   long long pos = pos0;
-  long long loglen = a.ctr->log_len;
-  long long acc_count = 0, neg = 0, ties = 0, exhausted = 0;
+  long long neg = 0, ties = 0, exhausted = 0;
   int m = 0;
+  int site_r = 0;                  // lane j: window index of flip j
+  double gam0_r = 0.0, gam1_r = 0.0;
 
   for (int t = 0; t < Wn; t++) {
-    const int x = cf[t];
+    const int x = __builtin_amdgcn_readlane(cfl, t);
     const int ix = x > 0 ? 1 : 0;
-    const double g0d = Gw[(size_t)t * LD + t];
-    const double g1d = flv > 1 ? Gw[(size_t)W * LD + (size_t)t * LD + t] : 0.0;
+    const double g0d = readlane_f64(d0, t);
+    const double g1d = flv > 1 ? readlane_f64(d1, t) : 0.0;
     const Proposal q = propose(flv, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
     const double p = q.p;
     bool accept;
@@ -153,7 +154,7 @@ __global__ __launch_bounds__(NT) void window_kernel(WindowArgs<GT> a) {
       double u;
       if (a.uniforms) {
         if (pos >= a.n_uniforms) { exhausted = 1; u = 1.0; }
-        else u = us[pos - pos0];
+        else u = readlane_f64(ul, (int)(pos - pos0));
       } else {
         u = counter_uniform(a.seed, (unsigned long long)pos);
       }
@@ -170,99 +171,103 @@ __global__ __launch_bounds__(NT) void window_kernel(WindowArgs<GT> a) {
 
     const double gam0 = gamma_of(q.D0, q.R0);
     const double gam1 = flv > 1 ? gamma_of(q.D1, q.R1) : 0.0;
-    // phase 1: new rank-1 vectors on the window and the bordering of K and Pinv
-    // (reads only: a wave that is behind still reads the diagonal of sites <= t).
-    for (int idx = tid; idx < flv * Wn; idx += NT) {
-      const int b = idx / Wn, r = idx - b * Wn;
-      const double* Gb = Gw + (size_t)b * W * LD;
-      tu[b * W + r] = (r == t ? 1.0 : 0.0) - Gb[(size_t)r * LD + t];
-      tv[b * W + r] = (b == 0 ? gam0 : gam1) * Gb[(size_t)t * LD + r];
-    }
-    for (int idx = tid; idx < flv * m; idx += NT) {
-      const int b = idx / m, l = idx - b * m;
-      const double* Ub = Uw + (size_t)b * W * LD + (size_t)t * LD;  // u_l'(s_t), l' < m
-      const double* Vb = Vw + (size_t)b * W * LD + (size_t)t * LD;  // v_l'(s_t)
-      const double* Kb = Kf + (size_t)b * W * LD;
-      const double* Pb = Pf + (size_t)b * W * LD;
-      double sk = 0.0, sp = 0.0;
-      for (int lp = l; lp < m; lp++) {
-        sk += Ub[lp] * Kb[(size_t)lp * LD + l];
-        sp += Pb[(size_t)l * LD + lp] * Vb[lp];
+    if (lane == m) { site_r = t; gam0_r = gam0; gam1_r = gam1; }
+    double* U0 = Uw + (size_t)m * LD;
+    double* U1 = Uw + (size_t)(W + m) * LD;
+    double* V0 = Vw + (size_t)m * LD;
+    double* V1 = Vw + (size_t)(W + m) * LD;
+    if (wave == (t & (NW - 1))) {           // owner of column t: u = e_t - G[:, t]
+      const int it = t / NW;
+      double c0 = 0.0, c1 = 0.0;
+#pragma unroll
+      for (int i = 0; i < kCPW; i++) {
+        c0 = i == it ? g0[i] : c0;
+        c1 = i == it ? g1[i] : c1;
       }
-      tk[b * W + l] = -(b == 0 ? gam0 : gam1) * sk;
-      tp[b * W + l] = sp;
+      if (lane < W) {
+        U0[lane] = (lane == t ? 1.0 : 0.0) - c0;
+        if (flv > 1) U1[lane] = (lane == t ? 1.0 : 0.0) - c1;
+      }
     }
+    if (lane == t) {                        // v = gamma G[t, :] on this wave's columns
+#pragma unroll
+      for (int i = 0; i < kCPW; i++) {
+        const int c = wave + NW * i;
+        if (c < W) {
+          V0[c] = gam0 * g0[i];
+          if (flv > 1) V1[c] = gam1 * g1[i];
+        }
+      }
+    }
+    if (tid == 0) a.conf[w0 + t] = (int8_t)(-x);
     __syncthreads();
-    // phase 2: rank-1 update of the window, record u, v and the bordered factors
-    for (int idx = tid; idx < flv * Wn * Wn; idx += NT) {
-      const int b = idx / (Wn * Wn), rem = idx - b * Wn * Wn, r = rem % Wn, c = rem / Wn;
-      Gw[(size_t)b * W * LD + (size_t)r * LD + c] -= tu[b * W + r] * tv[b * W + c];
+    const double u0 = lane < W ? U0[lane] : 0.0;
+    const double u1 = (lane < W && flv > 1) ? U1[lane] : 0.0;
+#pragma unroll
+    for (int i = 0; i < kCPW; i++) {
+      const int c = wave + NW * i;
+      if (c < W) {
+        g0[i] = fma(-u0, V0[c], g0[i]);
+        if (flv > 1) g1[i] = fma(-u1, V1[c], g1[i]);
+      }
     }
-    for (int idx = tid; idx < flv * Wn; idx += NT) {
-      const int b = idx / Wn, r = idx - b * Wn;
-      Uw[(size_t)b * W * LD + (size_t)r * LD + m] = tu[b * W + r];
-      Vw[(size_t)b * W * LD + (size_t)r * LD + m] = tv[b * W + r];
+    if (lane < W) {
+      d0 = fma(-u0, V0[lane], d0);
+      if (flv > 1) d1 = fma(-u1, V1[lane], d1);
     }
-    for (int idx = tid; idx < flv * (m + 1); idx += NT) {
-      const int b = idx / (m + 1), l = idx - b * (m + 1);
-      double* Kb = Kf + (size_t)b * W * LD;
-      double* Pb = Pf + (size_t)b * W * LD;
-      Kb[(size_t)m * LD + l] = l < m ? tk[b * W + l] : (b == 0 ? gam0 : gam1);
-      Pb[(size_t)l * LD + m] = l < m ? tp[b * W + l] : 1.0;
-    }
-    if (tid == 0) {
-      sites[m] = w0 + t;
-      a.conf[w0 + t] = (int8_t)(-x);
-    }
-    acc_count++;
     m++;
-    __syncthreads();
   }
-  __syncthreads();
 
-  // Y = Pinv K (m x m, zero-padded to W x W), C = (I - G)[:, S], R = G[S, :] (zero-padded)
-  for (int idx = tid; idx < flv * W * W; idx += NT) {
-    const int b = idx / (W * W), rem = idx - b * W * W, r = rem % W, c = rem / W;
-    double y = 0.0;
-    if (r < m && c < m) {
-      const double* Kb = Kf + (size_t)b * W * LD;
-      const double* Pb = Pf + (size_t)b * W * LD;
-      for (int l = (r > c ? r : c); l < m; l++) y += Pb[(size_t)r * LD + l] * Kb[(size_t)l * LD + c];
+  // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma by two triangular solves in registers: lane =
+  // row j, the wave's columns c = wsub + wpb q of block b.  The u, v columns were published
+  // before a barrier, so every wave sees them.
+  {
+    const int wpb = NW / flv;
+    const int b = wave / wpb, wsub = wave - b * wpb;
+    const double gj = b == 0 ? gam0_r : gam1_r;
+    const double* Ub = Uw + (size_t)b * W * LD;
+    const double* Vb = Vw + (size_t)b * W * LD;
+    double xq[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      const int c = wsub + wpb * q;
+      xq[q] = (c < m && lane == c) ? gj : 0.0;
     }
-    a.Y[(size_t)b * W * W + (size_t)r + (size_t)c * W] = (GT)y;
-  }
-  // C = (I - G)[:, S] (n x W) and R = G[S, :] (W x n), zero-padded beyond m
-  // Each thread first issues a batch of kSnap independent loads, then the stores.
-  constexpr int kSnap = 8;
-  for (int b = 0; b < flv; b++) {
-    const GT* __restrict__ Gb = G + (size_t)b * n * n;
-    GT* __restrict__ Cb = a.C + (size_t)b * n * W;
-    GT* __restrict__ Rb = a.R + (size_t)b * W * n;
-    for (int pr = tid; pr < n; pr += NT) {
-      for (int k0 = 0; k0 < W; k0 += kSnap) {
-        GT gc[kSnap], gr[kSnap];
+    // forward: K = (I + Gamma L)^-1 Gamma, column-oriented: x_j -= gamma_j u_l(s_j) x_l
+    for (int l = 0; l + 1 < m; l++) {
+      const double al = (lane > l && lane < m) ? gj * Ub[(size_t)l * LD + site_r] : 0.0;
 #pragma unroll
-        for (int q = 0; q < kSnap; q++) {
-          const int k = k0 + q;
-          const int sk = k < m ? sites[k] : -1;
-          gc[q] = sk >= 0 ? Gb[(size_t)pr + (size_t)sk * n] : GT(0);
-          gr[q] = sk >= 0 ? Gb[(size_t)sk + (size_t)pr * n] : GT(0);
-        }
-#pragma unroll
-        for (int q = 0; q < kSnap; q++) {
-          const int k = k0 + q;
-          if (k >= W) break;
-          const int sk = k < m ? sites[k] : -1;
-          Cb[(size_t)pr + (size_t)k * n] = sk >= 0 ? (pr == sk ? GT(1) : GT(0)) - gc[q] : GT(0);
-          Rb[(size_t)k + (size_t)pr * W] = gr[q];
-        }
+      for (int q = 0; q < kQ; q++) {
+        const int c = wsub + wpb * q;
+        if (c <= l) xq[q] = fma(-al, readlane_f64(xq[q], l), xq[q]);
       }
     }
+    // back: Y = (I - N)^-1 K: y_j += v_j(s_l) y_l for j < l
+    for (int l = m - 1; l >= 1; l--) {
+      const int sl = __builtin_amdgcn_readlane(site_r, l);
+      const int jl = lane < l ? lane : 0;   // keeps the address inside the block
+      const double nl = lane < l ? Vb[(size_t)jl * LD + sl] : 0.0;
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        const int c = wsub + wpb * q;
+        if (c < m) xq[q] = fma(nl, readlane_f64(xq[q], l), xq[q]);
+      }
+    }
+    GT* Yb = a.Y + (size_t)b * W * W;
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      const int c = wsub + wpb * q;
+      if (c < W && lane < W) Yb[(size_t)lane + (size_t)c * W] = (GT)xq[q];
+    }
+  }
+  if (wave == 0) {
+    if (lane < m) a.S[1 + lane] = w0 + site_r;
+    if (lane == 0) a.S[0] = m;
   }
   if (tid == 0) {
     ChainCounters* c = a.ctr;
     c->u_pos = pos;
-    c->accepted += acc_count;
+    c->accepted += m;
     c->proposed += Wn;
     c->negative += neg;
     c->ties += ties;
@@ -272,107 +277,147 @@ __global__ __launch_bounds__(NT) void window_kernel(WindowArgs<GT> a) {
   }
 }
 
-// G_b -= C_b (Y_b R_b) for one 64 x 64 tile of G per workgroup.  The tile's slice of
-// Z = Y R (W x 64) is formed in LDS first; the rank-W product is computed transposed,
-// tile^T = Z^T C^T, so accumulator element r of lane l is G[r0 + l%16][c0 + row(l, r)]:
-// quarter-waves touch 16 consecutive rows of a column (128-byte coalesced).
-template <typename GT>
-__global__ __launch_bounds__(256) void window_apply_kernel(GT* G, const GT* C, const GT* R, const GT* Y,
-                                                           int n, int W) {
+// Load K elements per thread with every load in flight before the first LDS store.
+template <int K, typename T, typename Ld, typename St>
+__device__ __forceinline__ void stage(int total, int tid, Ld ld, St st) {
+  T v[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int idx = tid + 256 * q;
+    v[q] = idx < total ? ld(idx) : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < K; q++) {
+    const int idx = tid + 256 * q;
+    if (idx < total) st(idx, v[q]);
+  }
+}
+
+// Gd_b = Gs_b - C_b (Y_b R_b) for one 64 x 64 tile per workgroup, with C = (I - Gs)[:, S]
+// and R = Gs[S, :] gathered from the source copy (the update writes the other copy, so
+// no tile can change a column or row that another workgroup still gathers).  The tile's
+// slice of Z = Y R (W16 x 64) is formed in LDS on the matrix cores; the rank-W16 product
+// is computed transposed, tile^T -= Z^T C^T, so accumulator element r of lane l is
+// G[r0 + l%16][c0 + row(l, r)]: quarter-waves touch 16 consecutive rows of a column.
+template <typename GT, int W16>
+__global__ __launch_bounds__(256) void window_apply_kernel(GT* __restrict__ Gd, const GT* __restrict__ Gs,
+                                                           const int* __restrict__ S,
+                                                           const GT* __restrict__ Y, int n, int W) {
   typedef typename V4<GT>::type acc_t;
   extern __shared__ __align__(16) unsigned char smem[];
   const int b = blockIdx.z;
   const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
-  const int W4 = (W + 3) & ~3;
-  GT* Ys = reinterpret_cast<GT*>(smem);          // W4 x W4   [k][l]
-  GT* Rs = Ys + W4 * W4;                          // W4 x 64   [l][col]
-  GT* Zs = Rs + W4 * 64;                          // W4 x 65   [k][col]
-  GT* Cs = Zs + W4 * 65;                          // 64 x W4+1 [row][k]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const GT* Yb = Y + (size_t)b * W * W;
-  const GT* Rb = R + (size_t)b * W * n;
-  const GT* Cb = C + (size_t)b * n * W;
-  for (int idx = tid; idx < W4 * W4; idx += 256) {
-    const int k = idx / W4, l = idx - k * W4;
-    Ys[idx] = (k < W && l < W) ? Yb[(size_t)k + (size_t)l * W] : GT(0);
-  }
-  for (int idx = tid; idx < W4 * 64; idx += 256) {
-    const int cc = idx / W4, l = idx - cc * W4;
-    const int gc = c0 + cc;
-    Rs[l * 64 + cc] = (l < W && gc < n) ? Rb[(size_t)l + (size_t)gc * W] : GT(0);
-  }
-  for (int idx = tid; idx < W4 * 64; idx += 256) {
-    const int k = idx / 64, rr = idx - k * 64;
-    const int gr = r0 + rr;
-    Cs[rr * (W4 + 1) + k] = (k < W && gr < n) ? Cb[(size_t)gr + (size_t)k * n] : GT(0);
-  }
-  __syncthreads();
-  for (int idx = tid; idx < W4 * 64; idx += 256) {
-    const int k = idx / 64, cc = idx - k * 64;
-    GT z = GT(0);
-    for (int l = 0; l < W4; l++) z += Ys[k * W4 + l] * Rs[l * 64 + cc];
-    Zs[k * 65 + cc] = z;
-  }
-  __syncthreads();
-  // each wave: 16-row x 64-column strip of the tile, as four 16x16 transposed MFMA tiles
   const int li = lane & 15, lk = lane >> 4;
-  GT* Gb = G + (size_t)b * n * n;
+  const size_t off = (size_t)b * n * n;
+  const GT* Gb = Gs + off;
+  GT* Gw = Gd + off;
   const int rr0 = wave * 16;
+  const int grow = r0 + rr0 + li;
   acc_t acc[4];
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int grow = r0 + rr0 + li;
+  for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int gcol = c0 + 16 * j + mfma_row<GT>(lane, r);
       acc[j][r] = (grow < n && gcol < n) ? Gb[(size_t)grow + (size_t)gcol * n] : GT(0);
     }
-  }
-  for (int k0 = 0; k0 < W4; k0 += 4) {
-    const int k = k0 + lk;
-    const GT bv = Cs[(rr0 + li) * (W4 + 1) + k];        // C^T[k][row]
+  const int m = S[0];
+  if (m > 0) {
+    GT* Ys = reinterpret_cast<GT*>(smem);   // W16 x W16 column-major: Ys[l * W16 + k] = Y[k][l]
+    GT* Rs = Ys + W16 * W16;                 // W16 x 64  [k][col]
+    GT* Zs = Rs + W16 * 64;                  // W16 x 65  [k][col]
+    GT* Cs = Zs + W16 * 65;                  // 64 x W16+1 [row][k]
+    const GT* Yb = Y + (size_t)b * W * W;
+    stage<(W16 * W16 + 255) / 256, GT>(W16 * W16, tid,
+        [&](int idx) { const int l = idx / W16, k = idx - l * W16;
+                       return (k < W && l < W) ? Yb[(size_t)k + (size_t)l * W] : GT(0); },
+        [&](int idx, GT v) { Ys[idx] = v; });
+    stage<(W16 * 64 + 255) / 256, GT>(W16 * 64, tid,
+        [&](int idx) { const int cc = idx / W16, k = idx - cc * W16, gc = c0 + cc;
+                       return (k < m && gc < n) ? Gb[(size_t)S[1 + k] + (size_t)gc * n] : GT(0); },
+        [&](int idx, GT v) { const int cc = idx / W16, k = idx - cc * W16; Rs[k * 64 + cc] = v; });
+    stage<(W16 * 64 + 255) / 256, GT>(W16 * 64, tid,
+        [&](int idx) { const int k = idx / 64, rr = idx - k * 64, gr = r0 + rr;
+                       if (k >= m || gr >= n) return GT(0);
+                       const int sk = S[1 + k];
+                       return (gr == sk ? GT(1) : GT(0)) - Gb[(size_t)gr + (size_t)sk * n]; },
+        [&](int idx, GT v) { const int k = idx / 64, rr = idx - k * 64; Cs[rr * (W16 + 1) + k] = v; });
+    __syncthreads();
+    // Z[:, 16 wave : 16 wave + 16] = Y R[:, same columns]
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const GT av = -Zs[k * 65 + 16 * j + li];          // Z^T[col][k]
-      acc[j] = mfma16(av, bv, acc[j]);
+    for (int i0 = 0; i0 < W16; i0 += 16) {
+      acc_t z = {0, 0, 0, 0};
+#pragma unroll
+      for (int k0 = 0; k0 < W16; k0 += 4) {
+        const GT av = Ys[(k0 + lk) * W16 + i0 + li];       // Y[i0 + li][k0 + lk]
+        const GT bv = Rs[(k0 + lk) * 64 + 16 * wave + li];  // R[k0 + lk][16 wave + li]
+        z = mfma16(av, bv, z);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Zs[(i0 + mfma_row<GT>(lane, r)) * 65 + 16 * wave + li] = z[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k0 = 0; k0 < W16; k0 += 4) {
+      const int k = k0 + lk;
+      const GT bv = Cs[(rr0 + li) * (W16 + 1) + k];       // C^T[k][row]
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const GT av = -Zs[k * 65 + 16 * j + li];          // Z^T[col][k]
+        acc[j] = mfma16(av, bv, acc[j]);
+      }
     }
   }
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int grow = r0 + rr0 + li;
+  for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int gcol = c0 + 16 * j + mfma_row<GT>(lane, r);
-      if (grow < n && gcol < n) Gb[(size_t)grow + (size_t)gcol * n] = acc[j][r];
+      if (grow < n && gcol < n) Gw[(size_t)grow + (size_t)gcol * n] = acc[j][r];
     }
-  }
 }
 
 template <typename GT>
-static hipError_t launch_apply_t(GT* G, const GT* C, const GT* R, const GT* Y, int n, int W, int flv,
-                                 hipStream_t s) {
-  const int W4 = (W + 3) & ~3;
-  const size_t lds = sizeof(GT) * ((size_t)W4 * W4 + W4 * 64 + W4 * 65 + 64 * (W4 + 1));
+static size_t apply_lds_bytes(int W16) {
+  return sizeof(GT) * ((size_t)W16 * W16 + W16 * 64 + W16 * 65 + 64 * (W16 + 1));
+}
+
+template <typename GT, int W16>
+static void launch_apply_w(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
+                           hipStream_t s) {
   dim3 grid((n + 63) / 64, (n + 63) / 64, flv);
-  hipLaunchKernelGGL(window_apply_kernel<GT>, grid, dim3(256), lds, s, G, C, R, Y, n, W);
+  hipLaunchKernelGGL((window_apply_kernel<GT, W16>), grid, dim3(256), apply_lds_bytes<GT>(W16), s, Gd, Gs, S,
+                     Y, n, W);
+}
+
+template <typename GT>
+static hipError_t launch_apply_t(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
+                                 hipStream_t s) {
+  if (W < 1 || W > 64) return hipErrorInvalidValue;
+  const int W16 = (W + 15) & ~15;
+  switch (W16) {
+    case 16: launch_apply_w<GT, 16>(Gd, Gs, S, Y, n, W, flv, s); break;
+    case 32: launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s); break;
+    case 48: launch_apply_w<GT, 48>(Gd, Gs, S, Y, n, W, flv, s); break;
+    default: launch_apply_w<GT, 64>(Gd, Gs, S, Y, n, W, flv, s); break;
+  }
   return hipGetLastError();
 }
-hipError_t launch_window_apply_f64(double* G, const double* C, const double* R, const double* Y, int n,
-                                   int W, int flv, hipStream_t s) {
-  return launch_apply_t(G, C, R, Y, n, W, flv, s);
-}
-hipError_t launch_window_apply_f32(float* G, const float* C, const float* R, const float* Y, int n, int W,
+hipError_t launch_window_apply_f64(double* Gd, const double* Gs, const int* S, const double* Y, int n, int W,
                                    int flv, hipStream_t s) {
-  return launch_apply_t(G, C, R, Y, n, W, flv, s);
+  return launch_apply_t(Gd, Gs, S, Y, n, W, flv, s);
+}
+hipError_t launch_window_apply_f32(float* Gd, const float* Gs, const int* S, const float* Y, int n, int W,
+                                   int flv, hipStream_t s) {
+  return launch_apply_t(Gd, Gs, S, Y, n, W, flv, s);
 }
 
-size_t window_lds_bytes(int W, int flv) {
-  const size_t LD = W + 1;
-  return (size_t)flv * W * LD * 8 * 5 + (size_t)flv * W * 8 * 4 + (size_t)W * 8 + (size_t)W * 4 + W + 16;
-}
+size_t window_lds_bytes(int W, int flv) { return (size_t)2 * flv * W * (W + 1) * 8; }
 
 int window_size(int n, int flv) {
-  int W = 48;
+  int W = 64;
+  while (W > 16 && W - 16 >= n) W -= 16;   // no wider than the lattice, rounded up to 16
   while (W > 8 && window_lds_bytes(W, flv) > 150 * 1024) W -= 8;
   return W;
 }
@@ -380,19 +425,29 @@ int window_size(int n, int flv) {
 template <typename GT>
 static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
   const size_t lds = window_lds_bytes(a.W, a.flv);
-  if (lds > 160 * 1024 || a.W < 1 || a.W > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((window_kernel<GT, 256>), dim3(1), dim3(256), lds, s, a);
+  if (lds > 160 * 1024 || a.W < 1 || a.W > 64 || a.flv < 1 || a.flv > 2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((window_kernel<GT>), dim3(1), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s) { return launch_window_t(a, s); }
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s) { return launch_window_t(a, s); }
 
 hipError_t local_kernels_init() {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<double, 256>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(window_kernel<float, 256>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double>),
+                       reinterpret_cast<const void*>(window_kernel<float>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 16>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 32>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 48>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 64>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 32>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 48>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 64>)};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace dqmc
