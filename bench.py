@@ -77,16 +77,52 @@ def timed_sweeps(ch, steps, warmup):
     return total, lat, acc
 
 
-def kernel_flops(kind, n, flv, accepted_per_launch=0.0):
-    """Algorithmic FLOP per launch of each kernel kind at block size n, flv blocks."""
+PANEL_DOUBLES, PANEL_MAX_W = 12 * 1024, 16   # panel rule of the UDT (qr.hip: udt_panel_width)
+
+
+def udt_panels(n):
+    """Panel starts and widths of the blocked UDT."""
+    out, p0 = [], 0
+    while p0 < n:
+        w = max(1, min(PANEL_DOUBLES // (n - p0), PANEL_MAX_W, n - p0))
+        out.append((p0, w))
+        p0 += w
+    return out
+
+
+def kernel_work(kind, n, flv, win, per_launch_accepted=0.0):
+    """Algorithmic FLOP per launch of each kernel kind at block size n with flv blocks.
+
+    gemm: one n^3 product per block.  udt_update: block-reflector updates, 4 rows w cols
+    flops per strip set, averaged over the trailing and the Q launches of one UDT.
+    udt_panel: Householder steps on the rows x w panel.  rdivp: triangular solve with n
+    right-hand rows.  local_apply: G - C (Y R) with m accepted flips per window.  local:
+    rank-1 updates of the window block plus the two m x m triangular solves.
+    """
     if kind in ("gemm_f64", "gemm_f32"):
-        return 2.0 * flv * n ** 3                    # one n x n x n product per block
-    if kind == "udt":
-        return flv * (8.0 / 3.0) * n ** 3            # Householder R (4/3 n^3) + explicit Q (4/3 n^3)
+        return 2.0 * flv * n ** 3
     if kind == "rdivp":
-        return flv * float(n) ** 3                   # triangular solve with n right-hand rows
+        return flv * float(n) ** 3
+    if kind == "udt_update":
+        pans = udt_panels(n)
+        tot, cnt = 0.0, 0
+        for p0, w in pans:
+            rows = n - p0
+            if p0 + w < n:
+                tot += 4.0 * rows * w * (n - p0 - w)
+                cnt += 1
+            tot += 4.0 * rows * w * rows
+            cnt += 1
+        return flv * tot / cnt
+    if kind == "udt_panel":
+        pans = udt_panels(n)
+        return flv * sum(2.0 * (n - p0) * w * w for p0, w in pans) / len(pans)
+    if kind == "local_apply":
+        m = per_launch_accepted
+        return 2.0 * flv * (n * n * m + m * m * n)
     if kind == "local":
-        return 2.0 * flv * n ** 2 * accepted_per_launch   # rank-1 updates of the accepted flips
+        m = per_launch_accepted
+        return flv * (2.0 * win * win * m + (2.0 / 3.0) * m ** 3)
     return 0.0
 
 
@@ -142,11 +178,13 @@ def main():
     acc_k = ch.stats()["accepted"] - acc_k0
     ch.set_kernel_timing(False)
     kernels = {}
+    win = 32 if n <= 32 else 64
+    windows = kt["local"][1]
     for kind, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
-        apl = acc_k / cnt if kind == "local" else 0.0
-        fl = kernel_flops(kind, n, flv, apl)
+        apl = acc_k / windows if kind in ("local", "local_apply") and windows else 0.0
+        fl = kernel_work(kind, n, flv, win, apl)
         avg_s = ms / cnt / 1e3
         kernels[kind] = {"launches_per_sweep": cnt / ksteps, "avg_us": avg_s * 1e6,
                          "ms_per_sweep": ms / ksteps, "flop_per_launch": fl,

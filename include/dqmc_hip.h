@@ -49,7 +49,7 @@ typedef struct {
   double alpha;     /* Hirsch coupling field.alpha (fields.jl:370-376, 419-425)             */
   int precision;    /* DQMC_PRECISION_*                                                     */
   int check_propagation_error; /* DQMCParameters.check_propagation_error                   */
-  int delay_depth;  /* delayed-update depth, 0 = automatic                                  */
+  int delay_depth;  /* sites per local-update window: <= 32 -> 32, else 64; 0 = automatic     */
   int device;       /* HIP device ordinal                                                   */
 } dqmc_config_t;
 
@@ -131,11 +131,13 @@ int dqmc_synchronize(dqmc_chain_t h);
  * the device time is accumulated per kernel kind.  Enabling resets the totals.          */
 #define DQMC_KERNEL_GEMM_F64 0 /* slice products, wraps, calculate_greens products */
 #define DQMC_KERNEL_GEMM_F32 1 /* mixed-precision wraps                            */
-#define DQMC_KERNEL_UDT 2      /* pivoted QR / UDT                                 */
+#define DQMC_KERNEL_UDT 2      /* UDT: column norms, Q initialisation, D/T output   */
 #define DQMC_KERNEL_RDIVP 3    /* triangular solve of calculate_greens             */
 #define DQMC_KERNEL_LOCAL 4    /* Metropolis decisions of one window of sites      */
 #define DQMC_KERNEL_LOCAL_APPLY 5 /* rank-m application of a window's flips to G    */
-#define DQMC_KERNEL_KINDS 6
+#define DQMC_KERNEL_UDT_PANEL 6  /* UDT: pivot choice + Householder panel (per panel) */
+#define DQMC_KERNEL_UDT_UPDATE 7 /* UDT: block-reflector trailing update / Q build   */
+#define DQMC_KERNEL_KINDS 8
 int dqmc_set_kernel_timing(dqmc_chain_t h, int enable);
 int dqmc_get_kernel_times(dqmc_chain_t h, double* total_ms, int64_t* launches, int kinds);
 

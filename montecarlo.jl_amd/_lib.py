@@ -48,7 +48,7 @@ EXPORTED = [
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
 ]
 
-KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local", "local_apply"]
+KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local", "local_apply", "udt_panel", "udt_update"]
 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")

@@ -1,6 +1,8 @@
 // Shared declarations of the gfx950 DQMC library (kernels + host orchestration).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <vector>
 #include <stdint.h>
 
 namespace dqmc {
@@ -82,6 +84,16 @@ struct UdtBatch {
   int apply_pivot;
 };
 hipError_t launch_udt(const UdtBatch& b, hipStream_t s);
+// The steps of launch_udt, for callers that time them separately: panel starts and widths,
+// column norms, panel p (single workgroup per item), strip reflections (mode 0: trailing
+// update of A from column c0 with residual norms; mode 1: Q accumulation into U from
+// column c0), D/T output.
+void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths);
+hipError_t udt_check(const UdtBatch& b);
+hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s);
+hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s);
+hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode, hipStream_t s);
+hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s);
 
 // rdivp: A <- A[:, pivot] * triu(T)^-1 with O as n*n scratch (qr.hip).
 struct RdivpItem {

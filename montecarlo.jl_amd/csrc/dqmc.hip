@@ -324,8 +324,34 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
     u.it[b].tfac = h->tfac + (size_t)b * (h->n + 16) * 16;
     u.it[b].swap = h->swp + (size_t)b * 32 * h->n;
   }
+  HIPCHK(h, udt_check(u));
+  std::vector<int> st, wd;
+  udt_plan(h->n, st, wd);
+  const int np = (int)st.size();
+  {
+    Timed t(h, DQMC_KERNEL_UDT);
+    HIPCHK(h, launch_udt_norms(u, h->st));
+  }
+  for (int pi = 0; pi < np; pi++) {
+    {
+      Timed t(h, DQMC_KERNEL_UDT_PANEL);
+      HIPCHK(h, launch_udt_panel(u, st[pi], wd[pi], pi == np - 1, h->st));
+    }
+    if (st[pi] + wd[pi] < h->n) {
+      Timed t(h, DQMC_KERNEL_UDT_UPDATE);
+      HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], st[pi] + wd[pi], 0, h->st));
+    }
+  }
+  {
+    Timed t(h, DQMC_KERNEL_UDT);
+    for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
+  }
+  for (int pi = np - 1; pi >= 0; pi--) {
+    Timed t(h, DQMC_KERNEL_UDT_UPDATE);
+    HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], st[pi], 1, h->st));
+  }
   Timed t(h, DQMC_KERNEL_UDT);
-  HIPCHK(h, launch_udt(u, h->st));
+  HIPCHK(h, launch_udt_finish(u, h->st));
   return DQMC_OK;
 }
 
@@ -735,8 +761,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   h->bweight[1] = std::exp(-(-2.0 * a * 1.0));
 
   h->win = window_size(h->n, h->flv);
-  if (cfg->delay_depth > 0) h->win = std::max(8, std::min(h->win, (cfg->delay_depth + 7) / 8 * 8));
-  if (h->win > 64) h->win = 64;
+  if (cfg->delay_depth > 0) h->win = cfg->delay_depth <= 32 ? 32 : 64;
 
   const size_t f = h->flv, nn = h->nn, n = h->n;
   int r = DQMC_OK;

@@ -79,7 +79,7 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Windowed form: the decisions of W <= 64 consecutive sites need only the W x W block of G.
+// Windowed form: the decisions of W consecutive sites need only the W x W block of G.
 // A window kernel (one workgroup per chain) runs the Metropolis decisions and the rank-1
 // updates on that block, and forms the factors of
 //     G_new = G - C Y R,   C = (I - G)[:, S],  R = G[S, :],
@@ -95,169 +95,196 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 // every wave publishes v on its columns (LDS), one barrier, then every wave applies the
 // rank-1 update to its columns and to its diagonal copy with the same fma, so the copies
 // stay bitwise equal to the block.  The published u, v columns are kept for the solves.
+// v is stored with the columns of one wave contiguous: position pc(c) = (c%4) W/4 + c/4.
 // ---------------------------------------------------------------------------------------
-template <typename GT>
+template <int W>
+__device__ __forceinline__ int vpos(int c) { return (c & 3) * (W / 4) + (c >> 2); }
+
+template <int FLV, int W>
+constexpr size_t window_lds() {
+  return (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4;
+}
+
+template <typename GT, int FLV, int W>
 __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
-  constexpr int NW = 4, kCPW = 16, kQ = 32;
+  constexpr int NW = 4, CPW = W / NW, LD = W + 1;
+  typedef double colvec __attribute__((ext_vector_type(CPW)));
   extern __shared__ __align__(16) unsigned char smem[];
-  const int n = a.n, flv = a.flv, W = a.W, w0 = a.w0;
-  const int Wn = n - w0 < W ? n - w0 : W;
-  const int LD = W + 1;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // LDS, per flavour block b: Uw[b][l][r] = u_l(r), Vw[b][l][c] = v_l(c)
+  // LDS, per flavour block b: Uw[b][l][r] = u_l(r);  Vw[b][l][vpos(c)] = G_l[s_l][c] (the
+  // raw row; v_l = gamma_l times it)
   double* Uw = reinterpret_cast<double*>(smem);
-  double* Vw = Uw + (size_t)flv * W * LD;
+  double* Vw = Uw + FLV * W * LD;
+  double* gam = Vw + FLV * W * LD;                        // [b][l]
+  int* sites = reinterpret_cast<int*>(gam + FLV * W);     // [l], window index of flip l
+  const int n = a.n, w0 = a.w0;
+  const int Wn = n - w0 < W ? n - w0 : W;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const GT* __restrict__ G = a.G;
   const size_t nn = (size_t)n * n;
 
   const long long pos0 = a.ctr->u_pos;
-  long long loglen = a.ctr->log_len;
+  const long long log0 = a.ctr->log_len;
+  // window block (registers), diagonal copy, field, this window's uniforms (lane k: u_{pos0+k})
+  const int lr = lane < Wn ? lane : Wn - 1;
   const bool rowok = lane < Wn;
-  double g0[kCPW], g1[kCPW];
+  colvec g0, g1;
 #pragma unroll
-  for (int i = 0; i < kCPW; i++) {
+  for (int i = 0; i < CPW; i++) {
     const int c = wave + NW * i;
+    const int cc = c < Wn ? c : Wn - 1;
+    const size_t o = (size_t)(w0 + lr) + (size_t)(w0 + cc) * n;
+    const double x0 = (double)G[o];
+    const double x1 = FLV > 1 ? (double)G[nn + o] : 0.0;
     const bool ok = rowok && c < Wn;
-    const size_t o = (size_t)(w0 + lane) + (size_t)(w0 + c) * n;
-    g0[i] = ok ? (double)G[o] : 0.0;
-    g1[i] = (ok && flv > 1) ? (double)G[nn + o] : 0.0;
+    g0[i] = ok ? x0 : 0.0;
+    g1[i] = ok ? x1 : 0.0;
   }
-  const size_t od = (size_t)(w0 + lane) * (n + 1);
+  const size_t od = (size_t)(w0 + lr) * (n + 1);
   double d0 = rowok ? (double)G[od] : 0.0;
-  double d1 = (rowok && flv > 1) ? (double)G[nn + od] : 0.0;
-  const int cfl = rowok ? (int)a.conf[w0 + lane] : 0;
+  double d1 = (FLV > 1 && rowok) ? (double)G[nn + od] : 0.0;
+  const int cfl = (int)a.conf[w0 + lr];
   double ul = 1.0;
-  if (a.uniforms && pos0 + lane < a.n_uniforms) ul = a.uniforms[pos0 + lane];
+  if (a.uniforms) {
+    if (pos0 + lane < a.n_uniforms) ul = a.uniforms[pos0 + lane];
+  } else {
+    ul = counter_uniform(a.seed, (unsigned long long)(pos0 + lane));
+  }
 
   const double df0 = a.d_first[0], df1 = a.d_first[1];
   const double ds0 = a.d_second[0], ds1 = a.d_second[1];
   const double bw0 = a.bweight[0], bw1 = a.bweight[1];
-  long long pos = pos0;
-  long long neg = 0, ties = 0, exhausted = 0;
+  int used = 0;                    // uniforms consumed in this window
+  int neg = 0, ties = 0;
   int m = 0;
-  int site_r = 0;                  // lane j: window index of flip j
-  double gam0_r = 0.0, gam1_r = 0.0;
+  // per lane: decision and p of site `lane`; site and gammas of flip `lane`
+  int acc_r = 0, site_r = 0;
+  double p_r = 0.0, gam0_r = 0.0, gam1_r = 0.0;
 
   for (int t = 0; t < Wn; t++) {
     const int x = __builtin_amdgcn_readlane(cfl, t);
     const int ix = x > 0 ? 1 : 0;
     const double g0d = readlane_f64(d0, t);
-    const double g1d = flv > 1 ? readlane_f64(d1, t) : 0.0;
-    const Proposal q = propose(flv, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
+    const double g1d = FLV > 1 ? readlane_f64(d1, t) : 0.0;
+    const Proposal q = propose(FLV, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
     const double p = q.p;
     bool accept;
-    if (p < 0.0) neg++;
+    neg += p < 0.0;
     if (p > 1.0) {
       accept = true;
-      if (p - 1.0 < 1e-12) ties++;
+      ties += (p - 1.0 < 1e-12);
     } else {
-      double u;
-      if (a.uniforms) {
-        if (pos >= a.n_uniforms) { exhausted = 1; u = 1.0; }
-        else u = readlane_f64(ul, (int)(pos - pos0));
-      } else {
-        u = counter_uniform(a.seed, (unsigned long long)pos);
-      }
-      pos++;
+      const double u = readlane_f64(ul, used);
+      used++;
       accept = u < p;
-      if (fabs(u - p) < 1e-12) ties++;
+      ties += (fabs(u - p) < 1e-12);
     }
-    if (tid == 0 && a.dec_log && loglen < a.log_cap) {
-      a.dec_log[loglen] = accept ? 1 : 0;
-      if (a.p_log) a.p_log[loglen] = p;
-    }
-    loglen++;
+    p_r = lane == t ? p : p_r;
+    acc_r = lane == t ? (int)accept : acc_r;
     if (!accept) continue;
 
-    const double gam0 = gamma_of(q.D0, q.R0);
-    const double gam1 = flv > 1 ? gamma_of(q.D1, q.R1) : 0.0;
-    if (lane == m) { site_r = t; gam0_r = gam0; gam1_r = gam1; }
-    double* U0 = Uw + (size_t)m * LD;
-    double* U1 = Uw + (size_t)(W + m) * LD;
-    double* V0 = Vw + (size_t)m * LD;
-    double* V1 = Vw + (size_t)(W + m) * LD;
+    double* U0 = Uw + m * LD;
+    double* U1 = Uw + (W + m) * LD;
+    double* V0 = Vw + m * LD;
+    double* V1 = Vw + (W + m) * LD;
     if (wave == (t & (NW - 1))) {           // owner of column t: u = e_t - G[:, t]
-      const int it = t / NW;
-      double c0 = 0.0, c1 = 0.0;
-#pragma unroll
-      for (int i = 0; i < kCPW; i++) {
-        c0 = i == it ? g0[i] : c0;
-        c1 = i == it ? g1[i] : c1;
-      }
+      const int it = t >> 2;
+      const double c0 = g0[it];
+      const double c1 = FLV > 1 ? g1[it] : 0.0;
       if (lane < W) {
-        U0[lane] = (lane == t ? 1.0 : 0.0) - c0;
-        if (flv > 1) U1[lane] = (lane == t ? 1.0 : 0.0) - c1;
+        const double e = lane == t ? 1.0 : 0.0;
+        U0[lane] = e - c0;
+        if (FLV > 1) U1[lane] = e - c1;
       }
     }
-    if (lane == t) {                        // v = gamma G[t, :] on this wave's columns
+    if (lane == t) {                        // row t on this wave's columns
 #pragma unroll
-      for (int i = 0; i < kCPW; i++) {
-        const int c = wave + NW * i;
-        if (c < W) {
-          V0[c] = gam0 * g0[i];
-          if (flv > 1) V1[c] = gam1 * g1[i];
-        }
+      for (int i = 0; i < CPW; i++) {
+        V0[wave * CPW + i] = g0[i];
+        if (FLV > 1) V1[wave * CPW + i] = g1[i];
       }
     }
-    if (tid == 0) a.conf[w0 + t] = (int8_t)(-x);
+    const double gam0 = gamma_of(q.D0, q.R0);
+    const double gam1 = FLV > 1 ? gamma_of(q.D1, q.R1) : 0.0;
+    site_r = lane == m ? t : site_r;
+    gam0_r = lane == m ? gam0 : gam0_r;
+    gam1_r = lane == m ? gam1 : gam1_r;
     __syncthreads();
-    const double u0 = lane < W ? U0[lane] : 0.0;
-    const double u1 = (lane < W && flv > 1) ? U1[lane] : 0.0;
+    // G -= (gamma u) row_t on this wave's columns and on the diagonal copy
+    const int lw = lane < W ? lane : W - 1;
+    const double u0 = lane < W ? gam0 * U0[lw] : 0.0;
+    const double u1 = (FLV > 1 && lane < W) ? gam1 * U1[lw] : 0.0;
 #pragma unroll
-    for (int i = 0; i < kCPW; i++) {
-      const int c = wave + NW * i;
-      if (c < W) {
-        g0[i] = fma(-u0, V0[c], g0[i]);
-        if (flv > 1) g1[i] = fma(-u1, V1[c], g1[i]);
-      }
+    for (int i = 0; i < CPW; i++) {
+      g0[i] = fma(-u0, V0[wave * CPW + i], g0[i]);
+      if (FLV > 1) g1[i] = fma(-u1, V1[wave * CPW + i], g1[i]);
     }
-    if (lane < W) {
-      d0 = fma(-u0, V0[lane], d0);
-      if (flv > 1) d1 = fma(-u1, V1[lane], d1);
-    }
+    d0 = fma(-u0, V0[vpos<W>(lw)], d0);
+    if (FLV > 1) d1 = fma(-u1, V1[vpos<W>(lw)], d1);
     m++;
   }
 
-  // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma by two triangular solves in registers: lane =
-  // row j, the wave's columns c = wsub + wpb q of block b.  The u, v columns were published
-  // before a barrier, so every wave sees them.
-  {
-    const int wpb = NW / flv;
-    const int b = wave / wpb, wsub = wave - b * wpb;
-    const double gj = b == 0 ? gam0_r : gam1_r;
-    const double* Ub = Uw + (size_t)b * W * LD;
-    const double* Vb = Vw + (size_t)b * W * LD;
-    double xq[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; q++) {
-      const int c = wsub + wpb * q;
-      xq[q] = (c < m && lane == c) ? gj : 0.0;
+  if (wave == 0) {
+    if (lane < m) {
+      sites[lane] = site_r;
+      gam[lane] = gam0_r;
+      if (FLV > 1) gam[W + lane] = gam1_r;
+    } else if (lane < W) {
+      sites[lane] = 0;
     }
-    // forward: K = (I + Gamma L)^-1 Gamma, column-oriented: x_j -= gamma_j u_l(s_j) x_l
-    for (int l = 0; l + 1 < m; l++) {
-      const double al = (lane > l && lane < m) ? gj * Ub[(size_t)l * LD + site_r] : 0.0;
+    if (lane < Wn) {
+      if (a.dec_log && log0 + lane < a.log_cap) {
+        a.dec_log[log0 + lane] = (unsigned char)acc_r;
+        if (a.p_log) a.p_log[log0 + lane] = p_r;
+      }
+      if (acc_r) a.conf[w0 + lane] = (int8_t)(-cfl);
+    }
+  }
+  __syncthreads();
+
+  // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma, wave b for flavour block b, lane = column c,
+  // rows in registers; four partial sums per dot product.
+  if (wave < FLV && m > 0) {
+    const int b = wave;
+    const double* Ub = Uw + b * W * LD;
+    const double* Vb = Vw + b * W * LD;
+    const double* gb = gam + b * W;
+    int st[W];
 #pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        const int c = wsub + wpb * q;
-        if (c <= l) xq[q] = fma(-al, readlane_f64(xq[q], l), xq[q]);
+    for (int l = 0; l < W; l++) st[l] = sites[l];
+    double xr[W];
+    // forward: x_j = gamma_j (delta_jc - sum_{l<j} u_l(s_j) x_l)
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+      if (j < m) {
+        double s4[4] = {lane == j ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int l = 0; l < j; l++) s4[l & 3] = fma(-Ub[l * LD + st[j]], xr[l], s4[l & 3]);
+        xr[j] = gb[j] * ((s4[0] + s4[1]) + (s4[2] + s4[3]));
+      } else {
+        xr[j] = 0.0;
       }
     }
-    // back: Y = (I - N)^-1 K: y_j += v_j(s_l) y_l for j < l
-    for (int l = m - 1; l >= 1; l--) {
-      const int sl = __builtin_amdgcn_readlane(site_r, l);
-      const int jl = lane < l ? lane : 0;   // keeps the address inside the block
-      const double nl = lane < l ? Vb[(size_t)jl * LD + sl] : 0.0;
+    // back: y_j = x_j + gamma_j sum_{l>j} G_j[s_j][s_l] y_l  (x_l = 0 and s_l = 0 for l >= m)
 #pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        const int c = wsub + wpb * q;
-        if (c < m) xq[q] = fma(nl, readlane_f64(xq[q], l), xq[q]);
+    for (int j = W - 2; j >= 0; j--) {
+      if (j < m - 1) {
+        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int lb = 0; lb < W; lb += 16) {
+          if (lb + 16 > j + 1 && lb < m) {
+#pragma unroll
+            for (int l = (lb > j + 1 ? lb : j + 1); l < lb + 16; l++)
+              s4[l & 3] = fma(Vb[j * LD + vpos<W>(st[l])], xr[l], s4[l & 3]);
+          }
+        }
+        xr[j] = fma(gb[j], (s4[0] + s4[1]) + (s4[2] + s4[3]), xr[j]);
       }
     }
+    // Y is stored row-major (Y[j][c] at j * W + c) so that lanes write consecutive columns
     GT* Yb = a.Y + (size_t)b * W * W;
+    if (lane < W) {
 #pragma unroll
-    for (int q = 0; q < kQ; q++) {
-      const int c = wsub + wpb * q;
-      if (c < W && lane < W) Yb[(size_t)lane + (size_t)c * W] = (GT)xq[q];
+      for (int j = 0; j < W; j++) Yb[(size_t)j * W + lane] = (GT)xr[j];
     }
   }
   if (wave == 0) {
@@ -266,13 +293,13 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
   }
   if (tid == 0) {
     ChainCounters* c = a.ctr;
-    c->u_pos = pos;
+    c->u_pos = pos0 + used;
     c->accepted += m;
     c->proposed += Wn;
     c->negative += neg;
     c->ties += ties;
-    c->exhausted |= exhausted;
-    c->log_len = loglen;
+    if (a.uniforms && pos0 + used > a.n_uniforms) c->exhausted = 1;
+    c->log_len = log0 + Wn;
     c->flushes += 1;
   }
 }
@@ -314,58 +341,102 @@ __global__ __launch_bounds__(256) void window_apply_kernel(GT* __restrict__ Gd, 
   GT* Gw = Gd + off;
   const int rr0 = wave * 16;
   const int grow = r0 + rr0 + li;
+  const int growc = grow < n ? grow : n - 1;
   acc_t acc[4];
 #pragma unroll
   for (int j = 0; j < 4; j++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const int gcol = c0 + 16 * j + mfma_row<GT>(lane, r);
-      acc[j][r] = (grow < n && gcol < n) ? Gb[(size_t)grow + (size_t)gcol * n] : GT(0);
+      acc[j][r] = Gb[(size_t)growc + (size_t)(gcol < n ? gcol : n - 1) * n];
     }
   const int m = S[0];
   if (m > 0) {
+    const int m16 = (m + 15) & ~15;
     GT* Ys = reinterpret_cast<GT*>(smem);   // W16 x W16 column-major: Ys[l * W16 + k] = Y[k][l]
     GT* Rs = Ys + W16 * W16;                 // W16 x 64  [k][col]
     GT* Zs = Rs + W16 * 64;                  // W16 x 65  [k][col]
     GT* Cs = Zs + W16 * 65;                  // 64 x W16+1 [row][k]
-    const GT* Yb = Y + (size_t)b * W * W;
-    stage<(W16 * W16 + 255) / 256, GT>(W16 * W16, tid,
-        [&](int idx) { const int l = idx / W16, k = idx - l * W16;
-                       return (k < W && l < W) ? Yb[(size_t)k + (size_t)l * W] : GT(0); },
-        [&](int idx, GT v) { Ys[idx] = v; });
-    stage<(W16 * 64 + 255) / 256, GT>(W16 * 64, tid,
-        [&](int idx) { const int cc = idx / W16, k = idx - cc * W16, gc = c0 + cc;
-                       return (k < m && gc < n) ? Gb[(size_t)S[1 + k] + (size_t)gc * n] : GT(0); },
-        [&](int idx, GT v) { const int cc = idx / W16, k = idx - cc * W16; Rs[k * 64 + cc] = v; });
-    stage<(W16 * 64 + 255) / 256, GT>(W16 * 64, tid,
-        [&](int idx) { const int k = idx / 64, rr = idx - k * 64, gr = r0 + rr;
-                       if (k >= m || gr >= n) return GT(0);
-                       const int sk = S[1 + k];
-                       return (gr == sk ? GT(1) : GT(0)) - Gb[(size_t)gr + (size_t)sk * n]; },
-        [&](int idx, GT v) { const int k = idx / 64, rr = idx - k * 64; Cs[rr * (W16 + 1) + k] = v; });
-    __syncthreads();
-    // Z[:, 16 wave : 16 wave + 16] = Y R[:, same columns]
+    // site of flip `lane` (lanes >= m: any valid row, masked below)
+    const int sk = S[1 + (lane < m ? lane : 0)];
+    {
+      // R[k][c0 + cc] = Gs[s_k][c0 + cc]: lane = k, cc = wave + 4 q
+      GT v[16];
 #pragma unroll
-    for (int i0 = 0; i0 < W16; i0 += 16) {
-      acc_t z = {0, 0, 0, 0};
-#pragma unroll
-      for (int k0 = 0; k0 < W16; k0 += 4) {
-        const GT av = Ys[(k0 + lk) * W16 + i0 + li];       // Y[i0 + li][k0 + lk]
-        const GT bv = Rs[(k0 + lk) * 64 + 16 * wave + li];  // R[k0 + lk][16 wave + li]
-        z = mfma16(av, bv, z);
+      for (int q = 0; q < 16; q++) {
+        const int gc = c0 + wave + 4 * q;
+        v[q] = Gb[(size_t)sk + (size_t)(gc < n ? gc : n - 1) * n];
       }
 #pragma unroll
-      for (int r = 0; r < 4; r++) Zs[(i0 + mfma_row<GT>(lane, r)) * 65 + 16 * wave + li] = z[r];
+      for (int q = 0; q < 16; q++) {
+        const int gc = c0 + wave + 4 * q;
+        if (lane < W16) Rs[lane * 64 + wave + 4 * q] = (lane < m && gc < n) ? v[q] : GT(0);
+      }
+    }
+    {
+      // C[r0 + rr][k] = delta - Gs[r0 + rr][s_k]: lane = rr, k = wave + 4 q (uniform per wave)
+      constexpr int QC = W16 / 4;
+      const int gr = r0 + lane;
+      const int grc = gr < n ? gr : n - 1;
+      GT v[QC];
+      int skq[QC];
+#pragma unroll
+      for (int q = 0; q < QC; q++) {
+        const int k = wave + 4 * q;
+        skq[q] = __builtin_amdgcn_readlane(sk, k < 64 ? k : 0);
+        v[q] = Gb[(size_t)grc + (size_t)skq[q] * n];
+      }
+#pragma unroll
+      for (int q = 0; q < QC; q++) {
+        const int k = wave + 4 * q;
+        Cs[lane * (W16 + 1) + k] = (k < m && gr < n) ? (gr == skq[q] ? GT(1) : GT(0)) - v[q] : GT(0);
+      }
+    }
+    {
+      // Y (row-major W x W in global): Ys[l * W16 + k] = Y[k][l]; lane = l, k = wave + 4 q
+      constexpr int QY = W16 / 4;
+      const int lc = lane < W ? lane : W - 1;
+      GT v[QY];
+#pragma unroll
+      for (int q = 0; q < QY; q++) {
+        const int k = wave + 4 * q;
+        v[q] = Y[(size_t)b * W * W + (size_t)(k < W ? k : W - 1) * W + lc];
+      }
+#pragma unroll
+      for (int q = 0; q < QY; q++) {
+        const int k = wave + 4 * q;
+        if (lane < W16) Ys[lane * W16 + k] = (k < W && lane < W) ? v[q] : GT(0);
+      }
+    }
+    __syncthreads();
+    // Z[:, 16 wave : 16 wave + 16] = Y R[:, same columns], rows and k up to m16
+#pragma unroll
+    for (int i0 = 0; i0 < W16; i0 += 16) {
+      if (i0 < m16) {
+        acc_t z = {0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < W16; k0 += 4) {
+          if (k0 < m16) {
+            const GT av = Ys[(k0 + lk) * W16 + i0 + li];       // Y[i0 + li][k0 + lk]
+            const GT bv = Rs[(k0 + lk) * 64 + 16 * wave + li];  // R[k0 + lk][16 wave + li]
+            z = mfma16(av, bv, z);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Zs[(i0 + mfma_row<GT>(lane, r)) * 65 + 16 * wave + li] = z[r];
+      }
     }
     __syncthreads();
 #pragma unroll
     for (int k0 = 0; k0 < W16; k0 += 4) {
-      const int k = k0 + lk;
-      const GT bv = Cs[(rr0 + li) * (W16 + 1) + k];       // C^T[k][row]
+      if (k0 < m16) {
+        const int k = k0 + lk;
+        const GT bv = Cs[(rr0 + li) * (W16 + 1) + k];       // C^T[k][row]
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const GT av = -Zs[k * 65 + 16 * j + li];          // Z^T[col][k]
-        acc[j] = mfma16(av, bv, acc[j]);
+        for (int j = 0; j < 4; j++) {
+          const GT av = -Zs[k * 65 + 16 * j + li];          // Z^T[col][k]
+          acc[j] = mfma16(av, bv, acc[j]);
+        }
       }
     }
   }
@@ -394,14 +465,9 @@ static void launch_apply_w(GT* Gd, const GT* Gs, const int* S, const GT* Y, int 
 template <typename GT>
 static hipError_t launch_apply_t(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
                                  hipStream_t s) {
-  if (W < 1 || W > 64) return hipErrorInvalidValue;
-  const int W16 = (W + 15) & ~15;
-  switch (W16) {
-    case 16: launch_apply_w<GT, 16>(Gd, Gs, S, Y, n, W, flv, s); break;
-    case 32: launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s); break;
-    case 48: launch_apply_w<GT, 48>(Gd, Gs, S, Y, n, W, flv, s); break;
-    default: launch_apply_w<GT, 64>(Gd, Gs, S, Y, n, W, flv, s); break;
-  }
+  if (W != 32 && W != 64) return hipErrorInvalidValue;
+  if (W == 32) launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s);
+  else launch_apply_w<GT, 64>(Gd, Gs, S, Y, n, W, flv, s);
   return hipGetLastError();
 }
 hipError_t launch_window_apply_f64(double* Gd, const double* Gs, const int* S, const double* Y, int n, int W,
@@ -413,35 +479,45 @@ hipError_t launch_window_apply_f32(float* Gd, const float* Gs, const int* S, con
   return launch_apply_t(Gd, Gs, S, Y, n, W, flv, s);
 }
 
-size_t window_lds_bytes(int W, int flv) { return (size_t)2 * flv * W * (W + 1) * 8; }
+size_t window_lds_bytes(int W, int flv) {
+  return (size_t)2 * flv * W * (W + 1) * 8 + (size_t)flv * W * 8 + (size_t)W * 4;
+}
 
 int window_size(int n, int flv) {
-  int W = 64;
-  while (W > 16 && W - 16 >= n) W -= 16;   // no wider than the lattice, rounded up to 16
-  while (W > 8 && window_lds_bytes(W, flv) > 150 * 1024) W -= 8;
-  return W;
+  (void)flv;
+  return n <= 32 ? 32 : 64;
+}
+
+template <typename GT, int FLV, int W>
+static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
+  hipLaunchKernelGGL((window_kernel<GT, FLV, W>), dim3(1), dim3(256), (window_lds<FLV, W>()), s, a);
 }
 
 template <typename GT>
 static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
-  const size_t lds = window_lds_bytes(a.W, a.flv);
-  if (lds > 160 * 1024 || a.W < 1 || a.W > 64 || a.flv < 1 || a.flv > 2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((window_kernel<GT>), dim3(1), dim3(256), lds, s, a);
+  if (a.flv < 1 || a.flv > 2 || (a.W != 32 && a.W != 64)) return hipErrorInvalidValue;
+  if (a.flv == 1) {
+    if (a.W == 32) launch_window_w<GT, 1, 32>(a, s); else launch_window_w<GT, 1, 64>(a, s);
+  } else {
+    if (a.W == 32) launch_window_w<GT, 2, 32>(a, s); else launch_window_w<GT, 2, 64>(a, s);
+  }
   return hipGetLastError();
 }
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s) { return launch_window_t(a, s); }
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s) { return launch_window_t(a, s); }
 
 hipError_t local_kernels_init() {
-  const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double>),
-                       reinterpret_cast<const void*>(window_kernel<float>),
-                       reinterpret_cast<const void*>(window_apply_kernel<double, 16>),
+  const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double, 1, 32>),
+                       reinterpret_cast<const void*>(window_kernel<double, 1, 64>),
+                       reinterpret_cast<const void*>(window_kernel<double, 2, 32>),
+                       reinterpret_cast<const void*>(window_kernel<double, 2, 64>),
+                       reinterpret_cast<const void*>(window_kernel<float, 1, 32>),
+                       reinterpret_cast<const void*>(window_kernel<float, 1, 64>),
+                       reinterpret_cast<const void*>(window_kernel<float, 2, 32>),
+                       reinterpret_cast<const void*>(window_kernel<float, 2, 64>),
                        reinterpret_cast<const void*>(window_apply_kernel<double, 32>),
-                       reinterpret_cast<const void*>(window_apply_kernel<double, 48>),
                        reinterpret_cast<const void*>(window_apply_kernel<double, 64>),
-                       reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 32>),
-                       reinterpret_cast<const void*>(window_apply_kernel<float, 48>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 64>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
