@@ -77,14 +77,14 @@ def timed_sweeps(ch, steps, warmup):
     return total, lat, acc
 
 
-PANEL_DOUBLES, PANEL_MAX_W = 12 * 1024, 16   # panel rule of the UDT (qr.hip: udt_panel_width)
-
-
 def udt_panels(n):
-    """Panel starts and widths of the blocked UDT."""
+    """Panel starts and widths of the blocked UDT (qr.hip: udt_panel_width)."""
     out, p0 = [], 0
     while p0 < n:
-        w = max(1, min(PANEL_DOUBLES // (n - p0), PANEL_MAX_W, n - p0))
+        rows, rpt = n - p0, 1
+        while rpt * 256 < rows:
+            rpt *= 2
+        w = max(1, min(64 // rpt, 16, rows))
         out.append((p0, w))
         p0 += w
     return out

@@ -241,16 +241,38 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
   }
   __syncthreads();
 
+  // Gather the m x m coefficient blocks, row-major, over the (now free) Uw/Vw storage:
+  //   Lc[b][j][l] = u_l(s_j) (l < j),  Nc[b][j][l] = G_j[s_j][s_l] (j < l), zero elsewhere.
+  double* Lc = Uw;                 // Lc[b] at 2 b W^2, Nc[b] at (2 b + 1) W^2
+  if (m > 0) {
+    constexpr int E = 2 * FLV * W * W / 256;
+    double gv[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+      const int e = tid + 256 * q;
+      const int mat = e / (W * W), rem = e - mat * W * W;
+      const int j = rem / W, l = rem - j * W;
+      const int b = mat >> 1;
+      // unconditional in-range loads (sites[] is 0 beyond m), then the mask
+      const double vl = Uw[(b * W + l) * LD + sites[j]];
+      const double vn = Vw[(b * W + j) * LD + vpos<W>(sites[l])];
+      const bool isl = (mat & 1) == 0;
+      const bool keep = isl ? (l < j && j < m) : (j < l && l < m);
+      gv[q] = keep ? (isl ? vl : vn) : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) Lc[tid + 256 * q] = gv[q];
+    __syncthreads();
+  }
+
   // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma, wave b for flavour block b, lane = column c,
   // rows in registers; four partial sums per dot product.
   if (wave < FLV && m > 0) {
     const int b = wave;
-    const double* Ub = Uw + b * W * LD;
-    const double* Vb = Vw + b * W * LD;
+    const double* Lb = Lc + 2 * b * W * W;
+    const double* Nb = Lc + (2 * b + 1) * W * W;
     const double* gb = gam + b * W;
-    int st[W];
-#pragma unroll
-    for (int l = 0; l < W; l++) st[l] = sites[l];
     double xr[W];
     // forward: x_j = gamma_j (delta_jc - sum_{l<j} u_l(s_j) x_l)
 #pragma unroll
@@ -258,13 +280,13 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
       if (j < m) {
         double s4[4] = {lane == j ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int l = 0; l < j; l++) s4[l & 3] = fma(-Ub[l * LD + st[j]], xr[l], s4[l & 3]);
+        for (int l = 0; l < j; l++) s4[l & 3] = fma(-Lb[j * W + l], xr[l], s4[l & 3]);
         xr[j] = gb[j] * ((s4[0] + s4[1]) + (s4[2] + s4[3]));
       } else {
         xr[j] = 0.0;
       }
     }
-    // back: y_j = x_j + gamma_j sum_{l>j} G_j[s_j][s_l] y_l  (x_l = 0 and s_l = 0 for l >= m)
+    // back: y_j = x_j + gamma_j sum_{l>j} G_j[s_j][s_l] y_l  (zero coefficients beyond m)
 #pragma unroll
     for (int j = W - 2; j >= 0; j--) {
       if (j < m - 1) {
@@ -274,7 +296,7 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
           if (lb + 16 > j + 1 && lb < m) {
 #pragma unroll
             for (int l = (lb > j + 1 ? lb : j + 1); l < lb + 16; l++)
-              s4[l & 3] = fma(Vb[j * LD + vpos<W>(st[l])], xr[l], s4[l & 3]);
+              s4[l & 3] = fma(Nb[j * W + l], xr[l], s4[l & 3]);
           }
         }
         xr[j] = fma(gb[j], (s4[0] + s4[1]) + (s4[2] + s4[3]), xr[j]);

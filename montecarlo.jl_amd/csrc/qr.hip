@@ -28,7 +28,7 @@ namespace dqmc {
 
 constexpr int kQrThreads = 256;
 constexpr int kQrMaxW = 16;
-constexpr int kPanelDoubles = 12 * 1024;  // 96 KB of panel storage
+constexpr int kPanelDoubles = 16 * 1024;  // 128 KB of panel storage
 
 // Wave64 sum of a double, result in every lane: butterflies inside each 16-lane row
 // with DPP (quad_perm xor 1, xor 2, row_ror 4, row_ror 8), then the four row sums are
@@ -65,78 +65,6 @@ __device__ __forceinline__ double vget(const double* P, int rows, int w, int i, 
   return i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
 }
 
-// X[p0:n, cb:cb+16] -= V op(T) V^T X[p0:n, cb:cb+16] with op(T) = T^T (trans) or T, on
-// the matrix cores, by one wave.  W = V^T X_strip and Y = op(T) W stay in registers: the
-// f64 16x16x4 accumulator element r of lane l is row l/16 + 4r, which is exactly the
-// B-operand element of k-chunk r, so W and Y feed the next products directly.  Loads are
-// issued kChunk k-steps at a time so that many are in flight.  When nrm is given, the
-// squared norms of the updated columns over rows >= p0 + w are written to it (the
-// residual norms the next panel pivots on).
-constexpr int kChunk = 32;
-
-__device__ void reflect_strip(double* __restrict__ X, int n, int p0, int rows, int cb, const double* P, int w,
-                              const double* Tf, bool trans, int lane, double* nrm) {
-  const int li = lane & 15, lk = lane >> 4;
-  const int col = cb + li;
-  const bool colok = col < n;
-  const double* xc = X + (size_t)(colok ? col : n - 1) * n + p0;
-  dx4 W = dx4{0, 0, 0, 0};
-  for (int kb = 0; kb < rows; kb += 4 * kChunk) {
-    double xb[kChunk];
-#pragma unroll
-    for (int q = 0; q < kChunk; q++) {
-      const int i = kb + 4 * q + lk;
-      xb[q] = (i < rows && colok) ? xc[i < rows ? i : rows - 1] : 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < kChunk; q++) {
-      const int i = kb + 4 * q + lk;
-      if (kb + 4 * q < rows) W = mfma16(vget(P, rows, w, i, li), xb[q], W);
-    }
-  }
-  dx4 Y = dx4{0, 0, 0, 0};
-#pragma unroll
-  for (int kc = 0; kc < 4; kc++) {
-    const int kk = 4 * kc + lk;
-    const double ta = trans ? Tf[kk * 16 + li] : Tf[li * 16 + kk];
-    Y = mfma16(ta, W[kc], Y);
-  }
-  double* xw = X + (size_t)(colok ? col : n - 1) * n + p0;
-  double ss = 0.0;
-  constexpr int kRB = 4;  // 16-row blocks per batch
-  for (int rb = 0; rb < rows; rb += 16 * kRB) {
-    dx4 acc[kRB];
-#pragma unroll
-    for (int q = 0; q < kRB; q++) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = rb + 16 * q + mfma_row<double>(lane, r);
-        acc[q][r] = (row < rows && colok) ? xw[row < rows ? row : rows - 1] : 0.0;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kRB; q++) {
-      const int r0 = rb + 16 * q;
-      if (r0 >= rows) break;
-#pragma unroll
-      for (int kc = 0; kc < 4; kc++) acc[q] = mfma16(-vget(P, rows, w, r0 + li, 4 * kc + lk), Y[kc], acc[q]);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int row = r0 + mfma_row<double>(lane, r);
-        if (row < rows && colok) {
-          xw[row] = acc[q][r];
-          if (row >= w) ss += acc[q][r] * acc[q][r];
-        }
-      }
-    }
-  }
-  if (nrm) {
-    ss += __shfl_xor(ss, 16, 64);
-    ss += __shfl_xor(ss, 32, 64);
-    if (lk == 0 && colok) nrm[col] = ss;
-  }
-}
-
 // The UDT runs as a sequence of launches per batch (one stream, so each sees the previous
 // one's results): column norms; per panel a single-workgroup panel kernel (pivot choice,
 // column moves, Householder factorisation of the panel in LDS, compact-WY factor T) and a
@@ -144,9 +72,17 @@ __device__ void reflect_strip(double* __restrict__ X, int n, int p0, int rows, i
 // per panel; finally D and T.  Per-item state between launches: it.tau holds the residual
 // norms, it.pivot the working permutation, it.tfac the T factor of each panel.
 
-// Panel width for `rows` remaining rows (the panel must fit kPanelDoubles).
+// Rows of the panel per thread of the panel kernel (256 threads): 1, 2, 4 or 8.
+static int udt_rows_per_thread(int rows) {
+  int R = 1;
+  while (R * 256 < rows) R *= 2;
+  return R;
+}
+
+// Panel width for `rows` remaining rows: the panel lives in registers (R rows x w columns
+// per thread, at most 64 values) and in LDS (rows x w <= kPanelDoubles).
 static int udt_panel_width(int rows) {
-  int w = kPanelDoubles / rows;
+  int w = 64 / udt_rows_per_thread(rows);
   if (w > kQrMaxW) w = kQrMaxW;
   if (w > rows) w = rows;
   if (w < 1) w = 1;
@@ -181,27 +117,35 @@ __global__ __launch_bounds__(256) void udt_norms_kernel(UdtBatch batch) {
 static size_t udt_panel_lds_bytes(int n) {
   size_t b = 0;
   b += (size_t)n * 8;                 // nrm
-  b += (size_t)kPanelDoubles * 8;     // panel
+  b += (size_t)kPanelDoubles * 8;     // panel (V for S = V^T V)
   b += 256 * 8 * 2;                   // Tf, S
-  b += 32 * 8;                        // scalars, taus
+  b += 32 * 8;                        // taus, scalars
+  b += 4 * 16 * 8 + 16 * 8;           // wave totals, row j
   b += (size_t)n * 4;                 // perm
   b += 3 * 32 * 4 + 16;               // moves, selection, misc
   return b;
 }
 
+// Householder panel of one item: pivot choice for the panel, column moves, factorisation
+// with the panel rows in registers (R rows x WM columns per thread), compact-WY factor T.
+// Per column step one block reduction gives ||x||^2 and x^T a_c for the later columns:
+// with v = (x + nu e_j) / (x_j + nu), v^T a_c = a_jc + (x_{>j}^T a_c{>j}) / (x_j + nu).
+template <int R, int WM>
 __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, int p0, int w, int last) {
   const UdtItem it = batch.it[blockIdx.x];
   const int n = batch.n;
   const int rows = n - p0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  constexpr int NW = kQrThreads / 64;
   extern __shared__ __align__(16) unsigned char smem[];
   double* nrm = reinterpret_cast<double*>(smem);
   double* P = nrm + n;
   double* Tf = P + kPanelDoubles;
   double* S = Tf + 256;
-  double* sc = S + 256;          // sc[0..15] taus of the panel, sc[16..] scalars
-  int* perm = reinterpret_cast<int*>(sc + 32);
+  double* sc = S + 256;          // sc[0..15] taus of the panel
+  double* red = P;               // [4][64][17] reduction transpose (P is written afterwards)
+  double* wt = sc + 32;          // [4][16]
+  double* rowj = wt + 4 * 16;    // [16]
+  int* perm = reinterpret_cast<int*>(rowj + 16);
   int* mv_dst = perm + n;
   int* mv_src = mv_dst + 32;
   int* selr = mv_src + 32;       // 16
@@ -215,202 +159,367 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   if (tid < kQrMaxW) selr[tid] = -1;
   __syncthreads();
 
-  // the w largest residual norms, ties to the lower index (reference: strict ">")
+  // the w largest residual norms, ties to the lower index (reference: strict ">"); the
+  // candidates are compared in batches of 16 so that many LDS reads are in flight
   for (int k = p0 + tid; k < n; k += kQrThreads) {
     const double v = nrm[k];
     int cnt = 0;
-#pragma unroll 8
-    for (int k2 = p0; k2 < n; k2++) {
-      const double u = nrm[k2];
-      cnt += (u > v) || (u == v && k2 < k);
+    for (int k0 = p0; k0 < n; k0 += 16) {
+      double u[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) u[q] = nrm[k0 + q < n ? k0 + q : n - 1];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int k2 = k0 + q;
+        cnt += k2 < n && ((u[q] > v) || (u[q] == v && k2 < k));
+      }
     }
     if (cnt < w) selr[cnt] = k;
   }
   __syncthreads();
 
-  // sequential-swap semantics -> list of column moves (thread 0, O(w^2))
-  if (tid == 0) {
-    bool ok = true;
-    for (int t = 0; t < w; t++) ok = ok && selr[t] >= p0 && selr[t] < n;
-    if (!ok)
-      for (int t = 0; t < w; t++) selr[t] = p0 + t;  // non-finite norms: keep the order
-    int dcol[kQrMaxW + 1], dslot[kQrMaxW + 1], nd = 0;
+  // Sequential swaps "slot p0+t <-> slot holding the t-th choice" simulated by wave 0 on
+  // the at most 2w slots involved: lane q < w is slot p0+q, lane w+q the slot of choice q
+  // when that lies outside the panel.  cont = original column now held by the lane's slot.
+  if (wave == 0) {
+    const int selv = lane < w ? selr[lane] : p0;
+    const bool ok = __ballot(lane < w && !(selv >= p0 && selv < n)) == 0ull;
+    const int selc = ok ? selv : p0 + lane;                    // lane t: t-th choice
+    // (all lanes take part in the shuffle; only then is the result selected)
+    const int cw = __shfl(selc, (lane + 64 - w) & 63, 64);
+    int slot = -1;
+    if (lane < w) slot = p0 + lane;
+    else if (lane < 2 * w) slot = cw >= p0 + w ? cw : -1;
+    int cont = slot;
     for (int t = 0; t < w; t++) {
-      const int c = selr[t];
-      const int target = p0 + t;
-      int slot_c = c, dc = -1;
-      for (int d = 0; d < nd; d++)
-        if (dcol[d] == c) { slot_c = dslot[d]; dc = d; }
-      if (slot_c == target) continue;
-      int content = target, dt = -1;
-      for (int d = 0; d < nd; d++)
-        if (dslot[d] == target) { content = dcol[d]; dt = d; }
-      if (dt >= 0) dslot[dt] = slot_c;
-      else { dcol[nd] = content; dslot[nd] = slot_c; nd++; }
-      if (dc >= 0) { dcol[dc] = dcol[nd - 1]; dslot[dc] = dslot[nd - 1]; nd--; }
-      const int tp = perm[target]; perm[target] = perm[slot_c]; perm[slot_c] = tp;
-      const double tn = nrm[target]; nrm[target] = nrm[slot_c]; nrm[slot_c] = tn;
+      const int c = __builtin_amdgcn_readlane(selc, t);
+      const unsigned long long hold = __ballot(slot >= 0 && cont == c);
+      // every choice is held by exactly one tracked slot; an empty ballot would mean a
+      // broken selection, and then the swap is skipped
+      const int src = hold ? __ffsll((long long)hold) - 1 : t;  // lane whose slot holds c
+      const int a_cont = __builtin_amdgcn_readlane(cont, t);      // content of slot p0+t
+      const int b_cont = __builtin_amdgcn_readlane(cont, src);
+      if (lane == t) cont = b_cont;
+      if (lane == src && src != t) cont = a_cont;
     }
-    int nm = 0;
-    for (int t = 0; t < w; t++)
-      if (selr[t] != p0 + t) { mv_dst[nm] = p0 + t; mv_src[nm] = selr[t]; nm++; }
-    for (int d = 0; d < nd; d++)
-      if (dcol[d] != dslot[d]) { mv_dst[nm] = dslot[d]; mv_src[nm] = dcol[d]; nm++; }
-    misc[0] = nm;
+    const bool moved = slot >= 0 && cont != slot;
+    const unsigned long long mm = __ballot(moved);
+    const int idx = __popcll(mm & ((1ull << lane) - 1ull));
+    if (moved) {
+      mv_dst[idx] = slot;
+      mv_src[idx] = cont;
+    }
+    if (lane == 0) misc[0] = __popcll(mm);
+    // panel column q (lane q < w) comes from original column cont
+    if (lane < w) selr[lane] = cont;
   }
   __syncthreads();
 
-  // physical column moves (whole columns), 256 rows at a time: every source element is
-  // read into registers before any destination is written
+  // physical column moves (whole columns) and the panel registers, both read from the
+  // source columns before anything is written
   const int nm = misc[0];
-  if (nm > 0) {
-    for (int r0 = 0; r0 < n; r0 += kQrThreads) {
-      const int r = r0 + tid;
-      const int rc = r < n ? r : n - 1;
-      double v[2 * kQrMaxW];
+  double a[R][WM];
 #pragma unroll
-      for (int q = 0; q < 2 * kQrMaxW; q++) {
-        const int src = q < nm ? mv_src[q] : 0;
-        v[q] = A[(size_t)rc + (size_t)src * n];
-      }
-      __syncthreads();
+  for (int k = 0; k < R; k++) {
+    const int i = tid + 256 * k;
+    const int ic = i < rows ? i : rows - 1;
 #pragma unroll
-      for (int q = 0; q < 2 * kQrMaxW; q++)
-        if (q < nm && r < n) A[(size_t)r + (size_t)mv_dst[q] * n] = v[q];
-      __syncthreads();
+    for (int c = 0; c < WM; c++) {
+      const int src = c < w ? selr[c] : selr[0];
+      a[k][c] = A[(size_t)(p0 + ic) + (size_t)src * n];
     }
   }
+  int pnew = 0;
+  if (tid < nm) pnew = perm[mv_src[tid]];
+  for (int r0 = 0; r0 < n; r0 += kQrThreads) {
+    const int rr = r0 + tid;
+    const int rc = rr < n ? rr : n - 1;
+    double v[2 * kQrMaxW];
+#pragma unroll
+    for (int q = 0; q < 2 * kQrMaxW; q++) {
+      const int src = q < nm ? mv_src[q] : p0;
+      v[q] = A[(size_t)rc + (size_t)src * n];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2 * kQrMaxW; q++)
+      if (q < nm && rr < n) A[(size_t)rr + (size_t)mv_dst[q] * n] = v[q];
+  }
+  if (tid < nm) perm[mv_dst[tid]] = pnew;
+  __syncthreads();
   for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
 
-  // panel factorisation in LDS
-  for (int idx = tid; idx < rows * w; idx += kQrThreads) {
-    const int jj = idx / rows, i = idx - jj * rows;
-    P[idx] = A[(size_t)(p0 + i) + (size_t)(p0 + jj) * n];
-  }
-  if (tid == 0) sc[16] = nrm[p0];
-  __syncthreads();
-  for (int jj = 0; jj < w; jj++) {
-    const int rj = jj;
-    const double normu = sc[16];
-    const double xi1 = P[rj + (size_t)rows * jj];
-    double tau = 0.0, nu = 0.0;
-    if (normu != 0.0) {
-      nu = copysign(sqrt(normu), xi1);
-      const double xi = xi1 + nu;
-      tau = xi / nu;
-      for (int i = rj + 1 + tid; i < rows; i += kQrThreads) P[i + (size_t)rows * jj] /= xi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      if (normu != 0.0) P[rj + (size_t)rows * jj] = -nu;
-      sc[jj] = tau;
-    }
-    for (int j2 = jj + 1 + wave; j2 < w; j2 += NW) {
-      double s = 0.0;
-      for (int i = rj + 1 + lane; i < rows; i += 64) s += P[i + (size_t)rows * jj] * P[i + (size_t)rows * j2];
-      s = wsum(s);
-      const double vA = tau * (P[rj + (size_t)rows * j2] + s);
-      for (int i = rj + 1 + lane; i < rows; i += 64) P[i + (size_t)rows * j2] -= P[i + (size_t)rows * jj] * vA;
-      if (lane == 0) P[rj + (size_t)rows * j2] -= vA;
-      if (j2 == jj + 1) {
-        double t = 0.0;
-        for (int i = rj + 1 + lane; i < rows; i += 64) {
-          const double x = P[i + (size_t)rows * j2];
-          t += x * x;
+  // factorisation, one block reduction per column
+  constexpr int G = 64 / WM;   // lane groups in the transpose reduction
+#pragma clang loop unroll(full)
+  for (int j = 0; j < WM; j++) {
+    if (j < w) {
+    double part[WM];
+#pragma unroll
+    for (int c = 0; c < WM; c++) part[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int i = tid + 256 * k;
+      if (i < rows && i >= j) {
+        const double x = a[k][j];
+        part[j] += x * x;
+        if (i > j) {
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
         }
-        t = wsum(t);
-        if (lane == 0) sc[16] = t;
+      }
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < WM; c++) rowj[c] = a[k][c];
       }
     }
+    double* rw = red + (size_t)(wave * 64) * 17;
+#pragma unroll
+    for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
+    __builtin_amdgcn_wave_barrier();
+    {
+      const int v = lane % WM, g = lane / WM;
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < WM; q++) s += rw[(g * WM + q) * 17 + v];
+#pragma unroll
+      for (int off = WM; off < 64; off *= 2) s += __shfl_xor(s, off, 64);
+      if (lane < WM) wt[wave * 16 + lane] = s;
+    }
     __syncthreads();
+    double tot = 0.0;
+    {
+      const int v = lane % WM;
+      tot = (wt[v] + wt[16 + v]) + (wt[32 + v] + wt[48 + v]);
+    }
+    const double normu = readlane_d(tot, j);
+    const double xi1 = rowj[j];
+    double tau = 0.0, nu = 0.0, xi = 1.0;
+    if (normu != 0.0) {
+      nu = copysign(sqrt(normu), xi1);
+      xi = xi1 + nu;
+      tau = xi / nu;
+    }
+    const double rxi = 1.0 / xi;
+    double wc[WM];
+#pragma unroll
+    for (int c = 0; c < WM; c++) wc[c] = c > j ? tau * (rowj[c] + readlane_d(tot, c) * rxi) : 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int i = tid + 256 * k;
+      if (normu != 0.0 && i < rows) {
+        if (i > j) {
+          const double vv = a[k][j] / xi;
+          a[k][j] = vv;
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
+        } else if (i == j) {
+          a[k][j] = -nu;
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) a[k][c] -= wc[c];
+        }
+      }
+    }
+    if (tid == 0) sc[j] = tau;
+    __syncthreads();   // rowj and wt are rewritten by the next step
+    }
   }
-  for (int idx = tid; idx < rows * w; idx += kQrThreads) {
-    const int jj = idx / rows, i = idx - jj * rows;
-    A[(size_t)(p0 + i) + (size_t)(p0 + jj) * n] = P[idx];
+  // panel -> A and LDS (V for S = V^T V)
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int i = tid + 256 * k;
+    if (i < rows) {
+#pragma unroll
+      for (int c = 0; c < WM; c++)
+        if (c < w) {
+          A[(size_t)(p0 + i) + (size_t)(p0 + c) * n] = a[k][c];
+          P[i + (size_t)rows * c] = a[k][c];
+          if (last && c == i) {   // D = |diag R| (0 -> 1) of this panel
+            const double x = fabs(a[k][c]);
+            it.D[p0 + i] = x == 0.0 ? 1.0 : x;
+          }
+        }
+    }
   }
 
   // compact-WY factor T (upper triangular, 16 x 16, zero beyond w):
-  // H_1..H_w = I - V T V^T.  S = V^T V on the matrix cores (wave 0), then lane i of
-  // wave 0 builds row i of T from its own earlier entries:
-  // T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j].
+  // H_1..H_w = I - V T V^T.  S = V^T V on the matrix cores (the four waves take every
+  // fourth k-step, partial sums reduced through LDS), then lane i of wave 0 builds row i
+  // of T from its own earlier entries, T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j],
+  // with S[k][j] taken from the accumulator registers by readlane (element k/4 of lane
+  // 16 (k%4) + j).
   __syncthreads();
-  if (wave == 0) {
+  {
     dx4 Sacc = dx4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < rows; k0 += 4) {
+    for (int k0 = 4 * wave; k0 < rows; k0 += 16) {
       const double v = vget(P, rows, w, k0 + (lane >> 4), lane & 15);
       Sacc = mfma16(v, v, Sacc);
     }
+    __syncthreads();   // every wave is done reading P: its space takes the partial sums
+    double* Sred = P;  // [wave][r][lane]
+    if (wave > 0) {
 #pragma unroll
-    for (int r = 0; r < 4; r++) S[mfma_row<double>(lane, r) * 16 + (lane & 15)] = Sacc[r];
-  }
-  __syncthreads();
-  if (tid < 16) {
-    const int i = tid;
-    double row[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      double t = 0.0;
-      if (i < w && j < w) {
-        if (j == i) t = sc[j];
-        else if (j > i) {
-          double s = 0.0;
-#pragma unroll
-          for (int k = 0; k < 16; k++)
-            if (k >= i && k < j) s += row[k] * S[k * 16 + j];
-          t = -sc[j] * s;
-        }
-      }
-      row[j] = t;
+      for (int r = 0; r < 4; r++) Sred[(wave * 4 + r) * 64 + lane] = Sacc[r];
     }
+    __syncthreads();
+    if (wave == 0) {
 #pragma unroll
-    for (int j = 0; j < 16; j++) it.tfac[(size_t)p0 * 16 + i * 16 + j] = row[j];
+      for (int r = 0; r < 4; r++)
+        Sacc[r] += (Sred[(4 + r) * 64 + lane] + Sred[(8 + r) * 64 + lane]) + Sred[(12 + r) * 64 + lane];
+    }
+    if (wave == 0) {
+      double row[16];
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+          if (k < j) {
+            const double skj = readlane_d(Sacc[k >> 2], (k & 3) * 16 + j);
+            acc += (k >= lane ? row[k] : 0.0) * skj;
+          }
+        }
+        const double tj = sc[j];
+        double t = 0.0;
+        if (lane < w && j < w) {
+          if (j == lane) t = tj;
+          else if (j > lane) t = -tj * acc;
+        }
+        row[j] = t;
+      }
+      if (lane < 16) {
+#pragma unroll
+        for (int j = 0; j < 16; j++) it.tfac[(size_t)p0 * 16 + lane * 16 + j] = row[j];
+      }
+    }
   }
   if (last) {
-    // D = |diag R| (0 -> 1); the diagonal of the earlier panels is final, this panel's
-    // diagonal is read from LDS
-    for (int i = tid; i < n; i += kQrThreads) {
-      const double r = i >= p0 ? P[(i - p0) + (size_t)rows * (i - p0)] : A[(size_t)i + (size_t)i * n];
-      const double x = fabs(r);
+    // D = |diag R| (0 -> 1) of the earlier panels (final since their launches)
+    for (int i = tid; i < p0; i += kQrThreads) {
+      const double x = fabs(A[(size_t)i + (size_t)i * n]);
       it.D[i] = x == 0.0 ? 1.0 : x;
     }
   }
 }
 
-// One wave per 16-column strip: mode 0 = trailing update of A with T^T (and the residual
-// norms of the updated columns), mode 1 = Q accumulation into U with T.
-static size_t udt_reflect_lds_bytes(int rows, int w) { return ((size_t)rows * w + 256) * 8; }
+// One workgroup per 16-column strip: X[p0:n, strip] -= V op(T) V^T X[p0:n, strip].
+// mode 0 = trailing update of A with T^T, plus the residual norms of the updated columns
+// over rows >= p0 + w; mode 1 = Q accumulation into U with T.  The four waves split the
+// 16-row blocks of the strip.  X is held in the f64 MFMA B/D layout (lane l: rows
+// l/16 + 4r of a block, column l%16), which is at once the B operand of W = V^T X and
+// the accumulator of X - V Y, so each block is loaded once; V is read in both
+// orientations.  All loads of a group of blocks are issued before the first product.
+constexpr int kReflRB = 4;   // 16-row blocks per wave and group
 
-__global__ __launch_bounds__(64) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int mode) {
+__device__ __forceinline__ double vpanel(const double* Vp, int n, int rows, int w, int i, int a) {
+  if (a >= w || i >= rows || i < a) return 0.0;
+  if (i == a) return 1.0;
+  return Vp[(size_t)i + (size_t)a * n];
+}
+
+__global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int mode) {
   const UdtItem it = batch.it[blockIdx.y];
   const int n = batch.n;
   const int rows = n - p0;
   const int cb = c0 + 16 * blockIdx.x;
-  const int lane = threadIdx.x;
-  extern __shared__ __align__(16) unsigned char smem[];
-  double* P = reinterpret_cast<double*>(smem);
-  double* Tf = P + (size_t)rows * w;
-  const double* __restrict__ A = it.A;
-  const int total = rows * w;
-  for (int base = 0; base < total; base += 64 * 16) {
-    double v[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  __shared__ double red[4][4][64];
+  __shared__ double nred[4][16];
+  const bool trans = mode == 0;
+  const double* Vp = it.A + (size_t)p0 * n + p0;
+  double* __restrict__ X = mode == 0 ? it.A : it.U;
+  const int col = cb + li;
+  const bool colok = col < n;
+  double* Xc = X + (size_t)(colok ? col : n - 1) * n + p0;
+  double ta[4];
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      int idx = base + 64 * q + lane;
-      idx = idx < total ? idx : total - 1;
-      const int jj = idx / rows, i = idx - jj * rows;
-      v[q] = A[(size_t)(p0 + i) + (size_t)(p0 + jj) * n];
+  for (int kc = 0; kc < 4; kc++) {
+    const int kk = 4 * kc + lk;
+    ta[kc] = trans ? it.tfac[(size_t)p0 * 16 + kk * 16 + li] : it.tfac[(size_t)p0 * 16 + li * 16 + kk];
+  }
+  const int nblk = (rows + 15) / 16;
+  const int ngroups = (nblk + 4 * kReflRB - 1) / (4 * kReflRB);
+  double xb[kReflRB][4], vb[kReflRB][4];
+  dx4 Wacc = dx4{0, 0, 0, 0};
+  for (int g = 0; g < ngroups; g++) {
+#pragma unroll
+    for (int q = 0; q < kReflRB; q++) {
+      const int blk = g * 4 * kReflRB + wave + 4 * q;
+#pragma unroll
+      for (int kc = 0; kc < 4; kc++) {
+        const int i = 16 * blk + 4 * kc + lk;
+        const int ic = i < rows ? i : rows - 1;
+        const double x = Xc[ic];
+        xb[q][kc] = (i < rows && colok) ? x : 0.0;
+        vb[q][kc] = vpanel(Vp, n, rows, w, i, li);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int idx = base + 64 * q + lane;
-      if (idx < total) P[idx] = v[q];
-    }
+    for (int q = 0; q < kReflRB; q++)
+#pragma unroll
+      for (int kc = 0; kc < 4; kc++) Wacc = mfma16(vb[q][kc], xb[q][kc], Wacc);
   }
 #pragma unroll
-  for (int q = 0; q < 4; q++) Tf[64 * q + lane] = it.tfac[(size_t)p0 * 16 + 64 * q + lane];
+  for (int r = 0; r < 4; r++) red[wave][r][lane] = Wacc[r];
   __syncthreads();
-  if (mode == 0) reflect_strip(it.A, n, p0, rows, cb, P, w, Tf, true, lane, it.tau);
-  else reflect_strip(it.U, n, p0, rows, cb, P, w, Tf, false, lane, nullptr);
+  dx4 Wt;
+#pragma unroll
+  for (int r = 0; r < 4; r++) Wt[r] = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
+  dx4 Y = dx4{0, 0, 0, 0};
+#pragma unroll
+  for (int kc = 0; kc < 4; kc++) Y = mfma16(ta[kc], Wt[kc], Y);
+
+  double ss = 0.0;
+  for (int g = 0; g < ngroups; g++) {
+    if (ngroups > 1) {
+#pragma unroll
+      for (int q = 0; q < kReflRB; q++) {
+        const int blk = g * 4 * kReflRB + wave + 4 * q;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+          const int i = 16 * blk + 4 * kc + lk;
+          const int ic = i < rows ? i : rows - 1;
+          const double x = Xc[ic];
+          xb[q][kc] = (i < rows && colok) ? x : 0.0;
+        }
+      }
+    }
+    double va[kReflRB][4];
+#pragma unroll
+    for (int q = 0; q < kReflRB; q++) {
+      const int blk = g * 4 * kReflRB + wave + 4 * q;
+#pragma unroll
+      for (int kc = 0; kc < 4; kc++) va[q][kc] = vpanel(Vp, n, rows, w, 16 * blk + li, 4 * kc + lk);
+    }
+#pragma unroll
+    for (int q = 0; q < kReflRB; q++) {
+      const int blk = g * 4 * kReflRB + wave + 4 * q;
+      if (16 * blk < rows) {
+        dx4 acc;
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] = xb[q][r];
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) acc = mfma16(-va[q][kc], Y[kc], acc);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int i = 16 * blk + lk + 4 * r;
+          if (i < rows && colok) {
+            Xc[i] = acc[r];
+            if (i >= w) ss += acc[r] * acc[r];
+          }
+        }
+      }
+    }
+  }
+  if (mode == 0) {
+    ss += __shfl_xor(ss, 16, 64);
+    ss += __shfl_xor(ss, 32, 64);
+    if (lk == 0) nred[wave][li] = ss;
+    __syncthreads();
+    if (tid < 16 && cb + tid < n)
+      it.tau[cb + tid] = (nred[0][tid] + nred[1][tid]) + (nred[2][tid] + nred[3][tid]);
+  }
 }
 
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
@@ -443,11 +552,15 @@ __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
 }
 
 hipError_t qr_kernels_init() {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(udt_panel_kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(udt_reflect_kernel),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  const void* fns[] = {reinterpret_cast<const void*>(udt_panel_kernel<1, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8>)};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths) {
@@ -474,16 +587,28 @@ hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <int R, int WM>
+static void launch_panel_t(const UdtBatch& b, int p0, int w, int last, hipStream_t s) {
+  hipLaunchKernelGGL((udt_panel_kernel<R, WM>), dim3(b.count), dim3(kQrThreads), udt_panel_lds_bytes(b.n), s, b,
+                     p0, w, last);
+}
+
 hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s) {
-  hipLaunchKernelGGL(udt_panel_kernel, dim3(b.count), dim3(kQrThreads), udt_panel_lds_bytes(b.n), s, b, p0, w,
-                     last);
+  const int rows = b.n - p0;
+  const int R = udt_rows_per_thread(rows);
+  if (w < 1 || w > udt_panel_width(rows)) return hipErrorInvalidValue;
+  switch (R) {
+    case 1: launch_panel_t<1, 16>(b, p0, w, last, s); break;
+    case 2: launch_panel_t<2, 16>(b, p0, w, last, s); break;
+    case 4: launch_panel_t<4, 16>(b, p0, w, last, s); break;
+    default: launch_panel_t<8, 8>(b, p0, w, last, s); break;
+  }
   return hipGetLastError();
 }
 
 hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode, hipStream_t s) {
   if (c0 >= b.n) return hipSuccess;
-  hipLaunchKernelGGL(udt_reflect_kernel, dim3((b.n - c0 + 15) / 16, b.count), dim3(64),
-                     udt_reflect_lds_bytes(b.n - p0, w), s, b, p0, w, c0, mode);
+  hipLaunchKernelGGL(udt_reflect_kernel, dim3((b.n - c0 + 15) / 16, b.count), dim3(256), 0, s, b, p0, w, c0, mode);
   return hipGetLastError();
 }
 
