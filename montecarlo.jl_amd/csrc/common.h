@@ -129,6 +129,7 @@ hipError_t launch_scale_cols(double* out, const double* in, const double* d, int
 
 // One-time kernel attribute setup (dynamic LDS above 64 KB).
 hipError_t qr_kernels_init();
+hipError_t gemm_kernels_init();
 hipError_t local_kernels_init();
 
 // Local update (local.hip)

@@ -29,6 +29,7 @@ hipError_t g_init_status = hipSuccess;
 void init_kernels() {
   hipError_t e = qr_kernels_init();
   if (e == hipSuccess) e = local_kernels_init();
+  if (e == hipSuccess) e = gemm_kernels_init();
   g_init_status = e;
 }
 }  // namespace
@@ -44,6 +45,11 @@ struct dqmc_chain_s {
   double d_first[2] = {0, 0}, d_second[2] = {0, 0}, bweight[2] = {1, 1};
   size_t nn = 0;
   std::vector<void*> allocs;
+  // Device memory of a chain is one allocation (2 MiB granular, so the GPU maps it with
+  // large pages), carved into 256-byte aligned buffers: sized in a first pass.
+  bool sizing = false;
+  size_t arena_need = 0, arena_off = 0, arena_cap = 0;
+  char* arena = nullptr;
 
   double *eTmh = nullptr, *eTph = nullptr, *eT2m = nullptr, *eT2p = nullptr, *dT = nullptr;
   double *G = nullptr, *Gx = nullptr, *Gchk = nullptr;
@@ -120,8 +126,23 @@ int generate_chunks(int length, int max_chunk, std::vector<int>& s, std::vector<
 
 template <typename T>
 int dalloc(Chain* h, T** p, size_t count) {
+  const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+  if (h->sizing) {
+    h->arena_need += bytes;
+    *p = nullptr;
+    return DQMC_OK;
+  }
+  if (h->arena) {
+    if (h->arena_off + bytes > h->arena_cap) {
+      h->err = "device arena exhausted";
+      return DQMC_E_ALLOC;
+    }
+    *p = reinterpret_cast<T*>(h->arena + h->arena_off);
+    h->arena_off += bytes;
+    return DQMC_OK;
+  }
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) {
     h->err = std::string("hipMalloc: ") + hipGetErrorString(e);
     return DQMC_E_ALLOC;
@@ -129,6 +150,28 @@ int dalloc(Chain* h, T** p, size_t count) {
   h->allocs.push_back(q);
   *p = reinterpret_cast<T*>(q);
   return DQMC_OK;
+}
+
+// Runs `list` (a sequence of dalloc calls) twice: once to size the arena, once to carve it.
+template <typename F>
+int arena_alloc(Chain* h, F list) {
+  h->sizing = true;
+  h->arena_need = 0;
+  int r = list();
+  h->sizing = false;
+  if (r != DQMC_OK) return r;
+  const size_t cap = (h->arena_need + ((size_t)2 << 20) - 1) & ~(((size_t)2 << 20) - 1);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, cap);
+  if (e != hipSuccess) {
+    h->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+    return DQMC_E_ALLOC;
+  }
+  h->allocs.push_back(q);
+  h->arena = reinterpret_cast<char*>(q);
+  h->arena_cap = cap;
+  h->arena_off = 0;
+  return list();
 }
 
 inline double* blk(double* base, const Chain* h, int b) { return base + (size_t)b * h->nn; }
@@ -766,6 +809,8 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   const size_t f = h->flv, nn = h->nn, n = h->n;
   int r = DQMC_OK;
   auto A = [&](auto** p, size_t count) { if (r == DQMC_OK) r = dalloc(h, p, count); };
+  r = arena_alloc(h, [&]() {
+  r = DQMC_OK;
   A(&h->eTmh, nn); A(&h->eTph, nn); A(&h->eT2m, nn); A(&h->eT2p, nn); A(&h->dT, nn);
   A(&h->G, f * nn); A(&h->Gx, f * nn); A(&h->Gchk, f * nn);
   A(&h->us, (size_t)(h->K + 1) * f * nn); A(&h->ts, (size_t)(h->K + 1) * f * nn);
@@ -785,6 +830,8 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   A(&h->ctr, 1);
   A(&h->errbuf, (size_t)h->errcap);
   A(&h->scal, 16);
+  return r;
+  });
   if (r != DQMC_OK) {
     g_create_error = h->err;
     for (void* p : h->allocs) (void)hipFree(p);
@@ -1192,11 +1239,15 @@ int la_make(LaCtx& x, int n, int nbuf) {
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) return DQMC_E_HIP;
   int r = DQMC_OK;
   auto A = [&](auto** p, size_t count) { if (r == DQMC_OK) r = dalloc(h, p, count); };
-  A(&h->tmp1, h->nn); A(&h->tmp2, h->nn); A(&h->cur, h->nn); A(&h->O, h->nn);
-  A(&h->Gx, h->nn); A(&h->Dl, (size_t)n); A(&h->piv, (size_t)n); A(&h->tau, (size_t)n);
-  A(&h->tfac, (size_t)(n + 16) * 16); A(&h->swp, (size_t)n * 32);
-  A(&h->Ul, h->nn); A(&h->Ur, h->nn); A(&h->Tl, h->nn); A(&h->Tr, h->nn); A(&h->Dr, (size_t)n);
-  A(&h->G, h->nn);
+  r = arena_alloc(h, [&]() {
+    r = DQMC_OK;
+    A(&h->tmp1, h->nn); A(&h->tmp2, h->nn); A(&h->cur, h->nn); A(&h->O, h->nn);
+    A(&h->Gx, h->nn); A(&h->Dl, (size_t)n); A(&h->piv, (size_t)n); A(&h->tau, (size_t)n);
+    A(&h->tfac, (size_t)(n + 16) * 16); A(&h->swp, (size_t)n * 32);
+    A(&h->Ul, h->nn); A(&h->Ur, h->nn); A(&h->Tl, h->nn); A(&h->Tr, h->nn); A(&h->Dr, (size_t)n);
+    A(&h->G, h->nn);
+    return r;
+  });
   (void)nbuf;
   return r;
 }

@@ -230,6 +230,8 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
       if (FLV > 1) gam[W + lane] = gam1_r;
     } else if (lane < W) {
       sites[lane] = 0;
+      gam[lane] = 0.0;                  // rows beyond m come out exactly zero
+      if (FLV > 1) gam[W + lane] = 0.0;
     }
     if (lane < Wn) {
       if (a.dec_log && log0 + lane < a.log_cap) {

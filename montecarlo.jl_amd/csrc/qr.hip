@@ -114,13 +114,36 @@ __global__ __launch_bounds__(256) void udt_norms_kernel(UdtBatch batch) {
   }
 }
 
+// (a, ia) comes before (b, ib): larger norm first, the lower column on ties.
+__device__ __forceinline__ bool pivot_before(double a, int ia, double b, int ib) {
+  return a > b || (a == b && ia < ib);
+}
+
+// Bitonic sort of one (norm, column) pair per lane of a wave into pivot order.
+__device__ __forceinline__ void bitonic64(double& kv, int& ki, int lane) {
+#pragma unroll
+  for (int size = 2; size <= 64; size *= 2) {
+#pragma unroll
+    for (int stride = size / 2; stride > 0; stride /= 2) {
+      const double pv = __shfl_xor(kv, stride, 64);
+      const int pi = __shfl_xor(ki, stride, 64);
+      const bool up = (lane & size) == 0 || size == 64;
+      const bool lower = (lane & stride) == 0;
+      const bool pb = pivot_before(pv, pi, kv, ki);
+      const bool take = lower == up ? pb : !pb;
+      if (take) { kv = pv; ki = pi; }
+    }
+  }
+}
+
 static size_t udt_panel_lds_bytes(int n) {
   size_t b = 0;
   b += (size_t)n * 8;                 // nrm
   b += (size_t)kPanelDoubles * 8;     // panel (V for S = V^T V)
   b += 256 * 8 * 2;                   // Tf, S
   b += 32 * 8;                        // taus, scalars
-  b += 4 * 16 * 8 + 16 * 8;           // wave totals, row j
+  b += 2 * (4 * 16 + 16) * 8;         // wave totals, row j (two buffers)
+  b += 4 * 16 * 16;                   // top-16 candidates of each wave
   b += (size_t)n * 4;                 // perm
   b += 3 * 32 * 4 + 16;               // moves, selection, misc
   return b;
@@ -143,9 +166,11 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   double* S = Tf + 256;
   double* sc = S + 256;          // sc[0..15] taus of the panel
   double* red = P;               // [4][64][17] reduction transpose (P is written afterwards)
-  double* wt = sc + 32;          // [4][16]
-  double* rowj = wt + 4 * 16;    // [16]
-  int* perm = reinterpret_cast<int*>(rowj + 16);
+  double* wt = sc + 32;          // [2][4][16] wave totals, by step parity
+  double* rowj = wt + 2 * 64;    // [2][16] row j, by step parity
+  double* candv = rowj + 32;     // [64] top-16 norms of each wave
+  int* candi = reinterpret_cast<int*>(candv + 64);   // [64] their columns
+  int* perm = candi + 64;
   int* mv_dst = perm + n;
   int* mv_src = mv_dst + 32;
   int* selr = mv_src + 32;       // 16
@@ -159,8 +184,26 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   if (tid < kQrMaxW) selr[tid] = -1;
   __syncthreads();
 
-  // the w largest residual norms, ties to the lower index (reference: strict ">"); the
-  // candidates are compared in batches of 16 so that many LDS reads are in flight
+  // the w largest residual norms, ties to the lower index (reference: strict ">")
+  if (rows <= kQrThreads) {
+    // bitonic sort of (norm, column) best-first inside each wave, the waves' top 16 merged
+    // by a second sort in wave 0
+    const int k = p0 + tid;
+    double kv = k < n ? nrm[k] : -HUGE_VAL;
+    int ki = k < n ? k : n + tid;
+    bitonic64(kv, ki, lane);
+    if (lane < 16) {
+      candv[wave * 16 + lane] = kv;
+      candi[wave * 16 + lane] = ki;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      kv = candv[lane];
+      ki = candi[lane];
+      bitonic64(kv, ki, lane);
+      if (lane < w) selr[lane] = ki;
+    }
+  } else
   for (int k = p0 + tid; k < n; k += kQrThreads) {
     const double v = nrm[k];
     int cnt = 0;
@@ -249,11 +292,16 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   __syncthreads();
   for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
 
-  // factorisation, one block reduction per column
-  constexpr int G = 64 / WM;   // lane groups in the transpose reduction
+  // factorisation, one block reduction per column: partial sums go through LDS within
+  // each wave (lane (v, g) adds value v of one group of lanes), the groups are combined
+  // with DPP inside quads / half rows, the four wave totals through LDS and one barrier;
+  // the row-j and total buffers alternate with the step parity, so no second barrier.
 #pragma clang loop unroll(full)
   for (int j = 0; j < WM; j++) {
     if (j < w) {
+    const int bsel = j & 1;
+    double* wtb = wt + bsel * 64;
+    double* rjb = rowj + bsel * 16;
     double part[WM];
 #pragma unroll
     for (int c = 0; c < WM; c++) part[c] = 0.0;
@@ -270,7 +318,7 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       }
       if (i == j) {
 #pragma unroll
-        for (int c = 0; c < WM; c++) rowj[c] = a[k][c];
+        for (int c = 0; c < WM; c++) rjb[c] = a[k][c];
       }
     }
     double* rw = red + (size_t)(wave * 64) * 17;
@@ -278,22 +326,25 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
     __builtin_amdgcn_wave_barrier();
     {
-      const int v = lane % WM, g = lane / WM;
-      double s = 0.0;
+      constexpr int GL = 64 / WM;            // lanes per value after the transpose
+      const int v = lane / GL, g = lane % GL;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < WM; q++) s += rw[(g * WM + q) * 17 + v];
-#pragma unroll
-      for (int off = WM; off < 64; off *= 2) s += __shfl_xor(s, off, 64);
-      if (lane < WM) wt[wave * 16 + lane] = s;
+      for (int q = 0; q < WM; q++) s4[q & 3] += rw[(g * WM + q) * 17 + v];
+      double sv = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      sv += dpp_mov<0xB1>(sv);             // quad_perm [1,0,3,2]
+      sv += dpp_mov<0x4E>(sv);             // quad_perm [2,3,0,1]
+      if (GL == 8) sv += dpp_mov<0x141>(sv);   // row_half_mirror: the other quad of 8
+      if (g == 0) wtb[wave * 16 + v] = sv;
     }
     __syncthreads();
-    double tot = 0.0;
+    double tot;
     {
       const int v = lane % WM;
-      tot = (wt[v] + wt[16 + v]) + (wt[32 + v] + wt[48 + v]);
+      tot = (wtb[v] + wtb[16 + v]) + (wtb[32 + v] + wtb[48 + v]);
     }
     const double normu = readlane_d(tot, j);
-    const double xi1 = rowj[j];
+    const double xi1 = rjb[j];
     double tau = 0.0, nu = 0.0, xi = 1.0;
     if (normu != 0.0) {
       nu = copysign(sqrt(normu), xi1);
@@ -303,13 +354,13 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     const double rxi = 1.0 / xi;
     double wc[WM];
 #pragma unroll
-    for (int c = 0; c < WM; c++) wc[c] = c > j ? tau * (rowj[c] + readlane_d(tot, c) * rxi) : 0.0;
+    for (int c = 0; c < WM; c++) wc[c] = c > j ? tau * (rjb[c] + readlane_d(tot, c) * rxi) : 0.0;
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const int i = tid + 256 * k;
       if (normu != 0.0 && i < rows) {
         if (i > j) {
-          const double vv = a[k][j] / xi;
+          const double vv = a[k][j] * rxi;
           a[k][j] = vv;
 #pragma unroll
           for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
@@ -321,9 +372,9 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       }
     }
     if (tid == 0) sc[j] = tau;
-    __syncthreads();   // rowj and wt are rewritten by the next step
     }
   }
+  __syncthreads();
   // panel -> A and LDS (V for S = V^T V)
 #pragma unroll
   for (int k = 0; k < R; k++) {

@@ -20,7 +20,7 @@ def graded(n, rng, decades=12):
     return np.asfortranarray(q1 @ np.diag(s) @ q2)
 
 
-@pytest.mark.parametrize("n", [4, 16, 36, 64, 100, 256])
+@pytest.mark.parametrize("n", [4, 16, 36, 64, 100, 256, 300, 576])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_gemm_fp64(hip, n, ta, tb):
     rng = np.random.default_rng(n + 10 * ta + 20 * tb)
@@ -32,7 +32,7 @@ def test_gemm_fp64(hip, n, ta, tb):
     assert np.abs(C - ref).max() <= 4 * n * np.finfo(float).eps * (np.abs(A).max() * np.abs(B).max() * n)
 
 
-@pytest.mark.parametrize("n", [16, 100, 256])
+@pytest.mark.parametrize("n", [16, 100, 256, 576])
 def test_gemm_fp32(hip, n):
     rng = np.random.default_rng(n)
     A = np.asfortranarray(rng.standard_normal((n, n)))
@@ -43,7 +43,7 @@ def test_gemm_fp32(hip, n):
     assert rel_maxdiff(C, ref) < 5e-6
 
 
-@pytest.mark.parametrize("n", [1, 2, 16, 36, 64, 144, 256])
+@pytest.mark.parametrize("n", [1, 2, 16, 36, 64, 144, 256, 300, 576, 1100])
 def test_udt_reconstruction(hip, n):
     rng = np.random.default_rng(7 * n)
     X = graded(n, rng, decades=8) if n > 2 else np.asfortranarray(rng.standard_normal((n, n)))
@@ -58,7 +58,7 @@ def test_udt_reconstruction(hip, n):
     assert sorted(piv.tolist()) == list(range(n))
 
 
-@pytest.mark.parametrize("n", [16, 64, 256])
+@pytest.mark.parametrize("n", [16, 64, 256, 576])
 def test_udt_matches_oracle_on_graded_input(hip, oracle, n):
     """Widely graded columns fix the pivot order, so GPU and oracle factors coincide."""
     rng = np.random.default_rng(n)
@@ -73,7 +73,7 @@ def test_udt_matches_oracle_on_graded_input(hip, oracle, n):
     assert np.abs(T - To).max() < 1e-11
 
 
-@pytest.mark.parametrize("n", [16, 100, 256])
+@pytest.mark.parametrize("n", [16, 100, 256, 576])
 def test_udt_dirty_upper(hip, n):
     rng = np.random.default_rng(3 * n)
     X = graded(n, rng, decades=6)
