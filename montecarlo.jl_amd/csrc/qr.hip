@@ -149,6 +149,163 @@ static size_t udt_panel_lds_bytes(int n) {
   return b;
 }
 
+// Column steps J.. of the single-wave panel factorisation (rows <= 256: lane l holds rows
+// l + 64 k).  A template recursion, so every register index is a compile-time constant.
+template <int J, int RT, int WM>
+__device__ __forceinline__ void panel_steps_wave(double (&a)[RT][WM], double (&taus)[WM], double* rw, int lane,
+                                                 int rows, int w) {
+  if constexpr (J < WM) {
+    constexpr int j = J;
+    taus[j] = 0.0;
+    if (j < w) {
+      double part[WM];
+#pragma unroll
+      for (int c = 0; c < WM; c++) part[c] = 0.0;
+#pragma unroll
+      for (int k = 0; k < RT; k++) {
+        const int i = lane + 64 * k;
+        if (i < rows && i >= j) {
+          const double x = a[k][j];
+          part[j] += x * x;
+          if (i > j) {
+#pragma unroll
+            for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
+      __builtin_amdgcn_wave_barrier();
+      const int v = lane >> 2, g = lane & 3;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 16; q++) s4[q & 3] += rw[(g * 16 + q) * 17 + v];
+      double sv = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      __builtin_amdgcn_wave_barrier();
+      sv += dpp_mov<0xB1>(sv);             // quad_perm [1,0,3,2]
+      sv += dpp_mov<0x4E>(sv);             // quad_perm [2,3,0,1]
+      const double normu = readlane_d(sv, 4 * j);
+      const double xi1 = readlane_d(a[0][j], j);   // row j < 64 lives in lane j, k = 0
+      double tau = 0.0, nu = 0.0, xi = 1.0;
+      if (normu != 0.0) {
+        nu = copysign(sqrt(normu), xi1);
+        xi = xi1 + nu;
+        tau = xi / nu;
+      }
+      const double rxi = 1.0 / xi;
+      double wc[WM];
+#pragma unroll
+      for (int c = 0; c < WM; c++)
+        wc[c] = c > j ? tau * (readlane_d(a[0][c], j) + readlane_d(sv, 4 * c) * rxi) : 0.0;
+#pragma unroll
+      for (int k = 0; k < RT; k++) {
+        const int i = lane + 64 * k;
+        if (normu != 0.0 && i < rows) {
+          if (i > j) {
+            const double vv = a[k][j] * rxi;
+            a[k][j] = vv;
+#pragma unroll
+            for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
+          } else if (i == j) {
+            a[k][j] = -nu;
+#pragma unroll
+            for (int c = j + 1; c < WM; c++) a[k][c] -= wc[c];
+          }
+        }
+      }
+      taus[j] = tau;
+    }
+    panel_steps_wave<J + 1, RT, WM>(a, taus, rw, lane, rows, w);
+  }
+}
+
+// Column steps J.. of the four-wave panel factorisation (thread t holds rows t + 256 k).
+// A template recursion, so every register index is a compile-time constant.  The row-j
+// and total buffers alternate with the step parity, so one barrier per step suffices.
+template <int J, int R, int WM>
+__device__ __forceinline__ void panel_steps_block(double (&a)[R][WM], double* red, double* wt, double* rowj,
+                                                  double* sc, int tid, int lane, int wave, int rows, int w) {
+  if constexpr (J < WM) {
+    constexpr int j = J;
+    if (j < w) {
+    const int bsel = j & 1;
+    double* wtb = wt + bsel * 64;
+    double* rjb = rowj + bsel * 16;
+    double part[WM];
+#pragma unroll
+    for (int c = 0; c < WM; c++) part[c] = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int i = tid + 256 * k;
+      if (i < rows && i >= j) {
+        const double x = a[k][j];
+        part[j] += x * x;
+        if (i > j) {
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
+        }
+      }
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < WM; c++) rjb[c] = a[k][c];
+      }
+    }
+    double* rw = red + (size_t)(wave * 64) * 17;
+#pragma unroll
+    for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
+    __builtin_amdgcn_wave_barrier();
+    {
+      constexpr int GL = 64 / WM;            // lanes per value after the transpose
+      const int v = lane / GL, g = lane % GL;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < WM; q++) s4[q & 3] += rw[(g * WM + q) * 17 + v];
+      double sv = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      sv += dpp_mov<0xB1>(sv);             // quad_perm [1,0,3,2]
+      sv += dpp_mov<0x4E>(sv);             // quad_perm [2,3,0,1]
+      if (GL == 8) sv += dpp_mov<0x141>(sv);   // row_half_mirror: the other quad of 8
+      if (g == 0) wtb[wave * 16 + v] = sv;
+    }
+    __syncthreads();
+    double tot;
+    {
+      const int v = lane % WM;
+      tot = (wtb[v] + wtb[16 + v]) + (wtb[32 + v] + wtb[48 + v]);
+    }
+    const double normu = readlane_d(tot, j);
+    const double xi1 = rjb[j];
+    double tau = 0.0, nu = 0.0, xi = 1.0;
+    if (normu != 0.0) {
+      nu = copysign(sqrt(normu), xi1);
+      xi = xi1 + nu;
+      tau = xi / nu;
+    }
+    const double rxi = 1.0 / xi;
+    double wc[WM];
+#pragma unroll
+    for (int c = 0; c < WM; c++) wc[c] = c > j ? tau * (rjb[c] + readlane_d(tot, c) * rxi) : 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int i = tid + 256 * k;
+      if (normu != 0.0 && i < rows) {
+        if (i > j) {
+          const double vv = a[k][j] * rxi;
+          a[k][j] = vv;
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
+        } else if (i == j) {
+          a[k][j] = -nu;
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) a[k][c] -= wc[c];
+        }
+      }
+    }
+    if (tid == 0) sc[j] = tau;
+    }
+    panel_steps_block<J + 1, R, WM>(a, red, wt, rowj, sc, tid, lane, wave, rows, w);
+  }
+}
+
 // Householder panel of one item: pivot choice for the panel, column moves, factorisation
 // with the panel rows in registers (R rows x WM columns per thread), compact-WY factor T.
 // Per column step one block reduction gives ||x||^2 and x^T a_c for the later columns:
@@ -261,15 +418,22 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   // physical column moves (whole columns) and the panel registers, both read from the
   // source columns before anything is written
   const int nm = misc[0];
-  double a[R][WM];
+  // R == 1 (rows <= 256): wave 0 alone factorises, lane l holding rows l + 64 k (k < 4);
+  // otherwise all 256 threads, thread t holding rows t + 256 k (k < R)
+  constexpr int RT = R == 1 ? 4 : R;         // rows per participating thread
+  constexpr int TP = R == 1 ? 64 : 256;      // participating threads
+  double a[RT][WM];
+  {
+    const int tl = tid & (TP - 1);   // threads beyond TP load a copy and never use it
 #pragma unroll
-  for (int k = 0; k < R; k++) {
-    const int i = tid + 256 * k;
-    const int ic = i < rows ? i : rows - 1;
+    for (int k = 0; k < RT; k++) {
+      const int i = tl + TP * k;
+      const int ic = i < rows ? i : rows - 1;
 #pragma unroll
-    for (int c = 0; c < WM; c++) {
-      const int src = c < w ? selr[c] : selr[0];
-      a[k][c] = A[(size_t)(p0 + ic) + (size_t)src * n];
+      for (int c = 0; c < WM; c++) {
+        const int src = c < w ? selr[c] : selr[0];
+        a[k][c] = A[(size_t)(p0 + ic) + (size_t)src * n];
+      }
     }
   }
   int pnew = 0;
@@ -292,94 +456,30 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   __syncthreads();
   for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
 
-  // factorisation, one block reduction per column: partial sums go through LDS within
-  // each wave (lane (v, g) adds value v of one group of lanes), the groups are combined
-  // with DPP inside quads / half rows, the four wave totals through LDS and one barrier;
-  // the row-j and total buffers alternate with the step parity, so no second barrier.
-#pragma clang loop unroll(full)
-  for (int j = 0; j < WM; j++) {
-    if (j < w) {
-    const int bsel = j & 1;
-    double* wtb = wt + bsel * 64;
-    double* rjb = rowj + bsel * 16;
-    double part[WM];
+  if constexpr (R == 1) {
+  // single-wave factorisation: per column one wave reduction (partial sums transposed
+  // through LDS, quads combined with DPP), row j read from its owner lane; no barrier
+  if (wave == 0) {
+    double taus[WM];
+    panel_steps_wave<0, RT, WM>(a, taus, red, lane, rows, w);
+    if (lane == 0) {
 #pragma unroll
-    for (int c = 0; c < WM; c++) part[c] = 0.0;
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      const int i = tid + 256 * k;
-      if (i < rows && i >= j) {
-        const double x = a[k][j];
-        part[j] += x * x;
-        if (i > j) {
-#pragma unroll
-          for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
-        }
-      }
-      if (i == j) {
-#pragma unroll
-        for (int c = 0; c < WM; c++) rjb[c] = a[k][c];
-      }
-    }
-    double* rw = red + (size_t)(wave * 64) * 17;
-#pragma unroll
-    for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
-    __builtin_amdgcn_wave_barrier();
-    {
-      constexpr int GL = 64 / WM;            // lanes per value after the transpose
-      const int v = lane / GL, g = lane % GL;
-      double s4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int q = 0; q < WM; q++) s4[q & 3] += rw[(g * WM + q) * 17 + v];
-      double sv = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-      sv += dpp_mov<0xB1>(sv);             // quad_perm [1,0,3,2]
-      sv += dpp_mov<0x4E>(sv);             // quad_perm [2,3,0,1]
-      if (GL == 8) sv += dpp_mov<0x141>(sv);   // row_half_mirror: the other quad of 8
-      if (g == 0) wtb[wave * 16 + v] = sv;
-    }
-    __syncthreads();
-    double tot;
-    {
-      const int v = lane % WM;
-      tot = (wtb[v] + wtb[16 + v]) + (wtb[32 + v] + wtb[48 + v]);
-    }
-    const double normu = readlane_d(tot, j);
-    const double xi1 = rjb[j];
-    double tau = 0.0, nu = 0.0, xi = 1.0;
-    if (normu != 0.0) {
-      nu = copysign(sqrt(normu), xi1);
-      xi = xi1 + nu;
-      tau = xi / nu;
-    }
-    const double rxi = 1.0 / xi;
-    double wc[WM];
-#pragma unroll
-    for (int c = 0; c < WM; c++) wc[c] = c > j ? tau * (rjb[c] + readlane_d(tot, c) * rxi) : 0.0;
-#pragma unroll
-    for (int k = 0; k < R; k++) {
-      const int i = tid + 256 * k;
-      if (normu != 0.0 && i < rows) {
-        if (i > j) {
-          const double vv = a[k][j] * rxi;
-          a[k][j] = vv;
-#pragma unroll
-          for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
-        } else if (i == j) {
-          a[k][j] = -nu;
-#pragma unroll
-          for (int c = j + 1; c < WM; c++) a[k][c] -= wc[c];
-        }
-      }
-    }
-    if (tid == 0) sc[j] = tau;
+      for (int j = 0; j < WM; j++) sc[j] = taus[j];
     }
   }
   __syncthreads();
+  } else {
+  // factorisation, one block reduction per column: partial sums go through LDS within
+  // each wave (lane (v, g) adds value v of one group of lanes), the groups are combined
+  // with DPP inside quads / half rows, the four wave totals through LDS and one barrier.
+  panel_steps_block<0, R, WM>(a, red, wt, rowj, sc, tid, lane, wave, rows, w);
+  __syncthreads();
+  }
   // panel -> A and LDS (V for S = V^T V)
 #pragma unroll
-  for (int k = 0; k < R; k++) {
-    const int i = tid + 256 * k;
-    if (i < rows) {
+  for (int k = 0; k < RT; k++) {
+    const int i = tid + TP * k;
+    if (tid < TP && i < rows) {
 #pragma unroll
       for (int c = 0; c < WM; c++)
         if (c < w) {
