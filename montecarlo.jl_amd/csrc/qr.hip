@@ -702,17 +702,6 @@ __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
   }
 }
 
-hipError_t qr_kernels_init() {
-  const void* fns[] = {reinterpret_cast<const void*>(udt_panel_kernel<1, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8>)};
-  for (const void* f : fns) {
-    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
-}
 
 void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths) {
   starts.clear();
@@ -853,11 +842,136 @@ __global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
   }
 }
 
+// rdivp for n <= kRdivpStripMaxN: one wave per 16-row strip.  The strip's pivoted input
+// is gathered into LDS once (X[k][r], column-major per strip) and every solved column
+// stays there, so a panel's coupling X_{<J} T_{<J,J} reads only T from global memory.
+constexpr int kRdivpStripMaxN = 1024;
+
+static size_t rdivp_strip_lds_bytes(int n) { return ((size_t)n * 17 + 2 * 16 * 17) * 8 + (size_t)n * 4; }
+
+__global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
+  const RdivpItem it = batch.it[blockIdx.y];
+  const int n = batch.n;
+  const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * 16;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* X = reinterpret_cast<double*>(smem);   // [n][17]
+  double* Ysh = X + (size_t)n * 17;              // [16][17]
+  double* Tsh = Ysh + 16 * 17;                   // [16][17]
+  int* pv = reinterpret_cast<int*>(Tsh + 16 * 17);
+  double* __restrict__ A = it.A;
+  const double* __restrict__ T = it.T;
+  const int rr = r0 + li;
+  const int rc = rr < n ? rr : n - 1;
+
+  for (int j0 = 0; j0 < n; j0 += 64 * 16) {
+    int t[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int j = j0 + lane + 64 * q;
+      t[q] = it.pivot[j < n ? j : n - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int j = j0 + lane + 64 * q;
+      if (j < n) pv[j] = t[q];
+    }
+  }
+  __syncthreads();
+  // O[r][j] = A[r][pivot[j]]: lane (row li, phase lk) takes columns lk + 4 q
+  for (int jb = 0; jb < n; jb += 128) {
+    double v[32];
+#pragma unroll
+    for (int q = 0; q < 32; q++) {
+      const int j = jb + lk + 4 * q;
+      v[q] = A[(size_t)rc + (size_t)pv[j < n ? j : n - 1] * n];
+    }
+#pragma unroll
+    for (int q = 0; q < 32; q++) {
+      const int j = jb + lk + 4 * q;
+      if (j < n) X[(size_t)j * 17 + li] = v[q];
+    }
+  }
+  __syncthreads();
+
+  for (int j0 = 0; j0 < n; j0 += 16) {
+    const int w = n - j0 < 16 ? n - j0 : 16;
+    const int cb = j0 + li;
+    const int cbc = cb < n ? cb : n - 1;
+    // diagonal block of T (out of range: identity)
+    double td[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int a = lk + 4 * q;
+      const int ta = j0 + a < n ? j0 + a : n - 1;
+      const double x = T[(size_t)ta + (size_t)cbc * n];
+      td[q] = (j0 + a < n && cb < n) ? x : (a == li ? 1.0 : 0.0);
+    }
+    dx4 acc = dx4{0, 0, 0, 0};
+    for (int kb = 0; kb < j0; kb += 128) {
+      double tb[32];
+#pragma unroll
+      for (int q = 0; q < 32; q++) {
+        const int k = kb + 4 * q + lk;
+        tb[q] = T[(size_t)(k < j0 ? k : 0) + (size_t)cbc * n];
+      }
+#pragma unroll
+      for (int q = 0; q < 32; q++) {
+        const int k = kb + 4 * q + lk;
+        if (kb + 4 * q < j0) acc = mfma16(X[(size_t)k * 17 + li], tb[q], acc);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) Tsh[(lk + 4 * q) * 17 + li] = td[q];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int row = mfma_row<double>(lane, r);
+      Ysh[row * 17 + li] = cb < n ? X[(size_t)cb * 17 + row] - acc[r] : 0.0;
+    }
+    __syncthreads();
+    if (lane < 16) {
+      double x[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        double v = Ysh[lane * 17 + c];
+#pragma unroll
+        for (int cp = 0; cp < 16; cp++)
+          if (cp < c) v -= x[cp] * Tsh[cp * 17 + c];
+        x[c] = v / Tsh[c * 17 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++)
+        if (c < w) X[(size_t)(j0 + c) * 17 + lane] = x[c];
+    }
+    __syncthreads();
+  }
+  if (rr < n) {
+    for (int j = lk; j < n; j += 4) A[(size_t)rr + (size_t)j * n] = X[(size_t)j * 17 + li];
+  }
+}
+
 hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
   if (b.count <= 0) return hipSuccess;
   if (b.count > kMaxUdtItems) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(rdivp_kernel, dim3((b.n + 63) / 64, b.count), dim3(256), 0, s, b);
+  if (b.n <= kRdivpStripMaxN)
+    hipLaunchKernelGGL(rdivp_strip_kernel, dim3((b.n + 15) / 16, b.count), dim3(64), rdivp_strip_lds_bytes(b.n), s,
+                       b);
+  else
+    hipLaunchKernelGGL(rdivp_kernel, dim3((b.n + 63) / 64, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
+}
+
+hipError_t qr_kernels_init() {
+  const void* fns[] = {reinterpret_cast<const void*>(udt_panel_kernel<1, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8>),
+                       reinterpret_cast<const void*>(rdivp_strip_kernel)};
+  for (const void* f : fns) {
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace dqmc
