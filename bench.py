@@ -126,6 +126,27 @@ def kernel_work(kind, n, flv, win, per_launch_accepted=0.0):
     return 0.0
 
 
+def kernel_bytes(kind, n, flv, win):
+    """Algorithmic HBM bytes per launch of the window kernel (the latency/memory-bound kernel):
+    the window block and its diagonal read, Y written, field column, flip list and log."""
+    if kind == "local":
+        return 2.0 * flv * win * win * 8 + flv * win * 8 + 3 * win + 4 * (win + 1)
+    return 0.0
+
+
+def measured_traffic(kernel_tag):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/dqmc_sweep_config_b_hbm_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "dqmc_sweep_config_b_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(kernel_tag + " traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pkg, model, par, field, sweeps):
     """The CPU oracle (FP64 restatement of the reference algorithm), one core."""
     sys.path.insert(0, ROOT)
@@ -151,6 +172,9 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--profile-slices", type=int, default=0,
+                    help="short run for hardware-counter profiles: build the stack, then this many "
+                         "(sweep_spatial + propagate) slice steps, and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: independent chains shard across GPUs with no "
@@ -160,6 +184,16 @@ def main():
     pkg = load_pkg()
     model, par, field = config_b(pkg)
     n, flv, M = len(model.l), field.flavors, par.slices
+
+    if args.profile_slices > 0:
+        ch = make_chain(pkg, model, par, field, "fp64")
+        for _ in range(args.profile_slices):
+            ch.sweep_spatial()
+            ch.propagate()
+        ch.synchronize()
+        print(json.dumps({"profile_slices": args.profile_slices, "state": ch.state()}))
+        ch.close()
+        return
 
     # ---- FP64 headline ------------------------------------------------------------------
     ch = make_chain(pkg, model, par, field, "fp64")
@@ -190,12 +224,23 @@ def main():
                          "ms_per_sweep": ms / ksteps, "flop_per_launch": fl,
                          "tflops": fl / avg_s / 1e12 if avg_s > 0 else 0.0}
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_sweep"])
-    peak = PEAK["fp32_mfma_tflops"] if dom == "gemm_f32" else PEAK["fp64_mfma_tflops"]
-    achieved = kernels[dom]["tflops"]
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "avg_launch_us": kernels[dom]["avg_us"],
-                "flop_per_launch": kernels[dom]["flop_per_launch"]}
+    if dom == "local":
+        # the window kernel is a latency chain of Metropolis decisions over a block held in
+        # registers: its bound is memory (HBM bytes in and out per launch)
+        nbytes = kernel_bytes("local", n, flv, win)
+        achieved = nbytes / (kernels[dom]["avg_us"] * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "kernel": "window_kernel", "achieved": achieved,
+                    "peak": PEAK["hbm_gbs"], "unit": "GB/s", "frac": achieved / PEAK["hbm_gbs"],
+                    "traffic": measured_traffic("window_kernel"),
+                    "avg_launch_us": kernels[dom]["avg_us"], "bytes_per_launch": nbytes,
+                    "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dqmc_sweep_config_b_hbm_traffic.md"}
+    else:
+        peak = PEAK["fp32_mfma_tflops"] if dom == "gemm_f32" else PEAK["fp64_mfma_tflops"]
+        achieved = kernels[dom]["tflops"]
+        roofline = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
+                    "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                    "avg_launch_us": kernels[dom]["avg_us"],
+                    "flop_per_launch": kernels[dom]["flop_per_launch"]}
     ch.close()
 
     # ---- mixed precision ---------------------------------------------------------------

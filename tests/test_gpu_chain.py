@@ -154,3 +154,28 @@ def test_config_b_sweep_parity(pkg, hip, oracle):
         assert mism == 0
         assert np.array_equal(g.get_field(), o.get_field())
         assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+
+
+@pytest.mark.parametrize("case", ["repulsive", "attractive_mu1"])
+def test_dqmc_driver_on_gpu_matches_ed(pkg, case):
+    """The reference's finite-U ED test (test/ED/ED_tests.jl:402-484) through the host driver
+    on the HIP chain: 2x2, beta=1, dtau=0.1, safe_mult=5, 5k + 5k sweeps; atol 3.05 dtau^2,
+    rtol 2 dtau^2."""
+    import json
+    import os
+    from conftest import ROOT
+    ed = json.load(open(os.path.join(ROOT, "tests", "golden", "ed_2x2.json")))[case]
+    model = pkg.HubbardModel(2, 2, U=ed["U"], mu=ed["mu"], t=1.0)
+    mc = pkg.DQMC(model, beta=1.0, delta_tau=0.1, safe_mult=5, thermalization=5000, sweeps=5000,
+                  measure_rate=1, seed=77)
+    assert mc.run(verbose=False) == pkg.ExitCode.SUCCESS
+    ms = mc.measurements["equal_time"]
+    atol, rtol = 3.05 * 0.01, 2 * 0.01
+    close = lambda a, b: np.all(np.abs(np.asarray(a) - np.asarray(b)) <= atol + rtol * np.abs(np.asarray(b)))
+    assert close(ms.mean("total_energy"), ed["energy"])
+    G = ms.mean("greens")
+    Ged = np.asarray(ed["greens"])
+    for b in range(G.shape[0]):
+        assert close(G[b], Ged[b * 4:(b + 1) * 4, b * 4:(b + 1) * 4])
+    assert close(ms.mean("occupation"), 1 - np.diag(Ged)[: 4 * G.shape[0]])
+    assert close(ms.mean("double_occupancy"), ed["double_occupancy"])
