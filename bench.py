@@ -172,6 +172,8 @@ def main():
     ap.add_argument("--cpu-sweeps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-mixed", action="store_true")
+    ap.add_argument("--batch", type=int, default=8,
+                    help="also time this many config-B chains swept together (0: skip)")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="short run for hardware-counter profiles: build the stack, then this many "
                          "(sweep_spatial + propagate) slice steps, and exit")
@@ -257,6 +259,32 @@ def main():
                  "speedup_vs_fp64": (args.steps / mt) / sps}
         cm.close()
 
+    # ---- batched form: several independent chains, one stream and host thread each ---------
+    batched = None
+    if args.batch > 1:
+        from montecarlo_jl_amd import _lib as L
+        chains = []
+        for k in range(args.batch):
+            f = model.choose_field()(par, model)
+            f.rand(1 + k)
+            c = L.HIPChain(n, flv, M, par.safe_mult, par.delta_tau, f.alpha)
+            c.set_hopping(model.hopping_matrix())
+            c.set_field(f.conf)
+            c.set_seed(12345 + k)
+            c.build_stack()
+            chains.append(c)
+        L.batch_sweep(chains, 1)
+        bsteps = max(2, args.steps // 4)
+        t0 = time.perf_counter()
+        L.batch_sweep(chains, bsteps)
+        bt = time.perf_counter() - t0
+        batched = {"chains": args.batch, "sweeps_per_chain": bsteps,
+                   "aggregate_sweeps_per_s": args.batch * bsteps / bt,
+                   "per_chain_sweeps_per_s": bsteps / bt,
+                   "speedup_vs_one_chain": (args.batch * bsteps / bt) / sps}
+        for c in chains:
+            c.close()
+
     # ---- CPU baseline --------------------------------------------------------------------
     cpu = None
     if not args.skip_cpu:
@@ -281,6 +309,7 @@ def main():
         "propagation_error_max": st["propagation_error_max"],
         "propagation_error_last": st["propagation_error_last"],
         "mixed": mixed,
+        "batched": batched,
         "roofline": roofline,
         "kernels": kernels,
         "kernel_timing_region": {"sweeps": ksteps, "ms_per_sweep_with_events": t_events / ksteps * 1e3},

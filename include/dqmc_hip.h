@@ -111,6 +111,13 @@ int dqmc_sweep_spatial(dqmc_chain_t h, int64_t* accepted);
 /* nsweeps x local_sweep (local_updates.jl:7-14): 2M x (sweep_spatial; propagate).      */
 int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted);
 
+/* Batched form: `count` distinct chains sweep `nsweeps` times each, concurrently (one host
+ * thread and one stream per chain), each exactly as dqmc_sweep would alone.  accepted[i]
+ * (optional) receives chain i's accepted flips.  No reference counterpart: the reference
+ * runs one chain per Julia process (run!, DQMC.jl:144-394); this is the many-chain form of
+ * update(::LocalSweep) (local_updates.jl:82) for independent chains. */
+int dqmc_batch_sweep(dqmc_chain_t* chains, int count, int nsweeps, int64_t* accepted);
+
 /* mc.stack.greens (physical = 0) or greens(mc) = e^{dtau T/2} G e^{-dtau T/2}
  * (physical = 1, greens.jl:87-125); n x (flavors*n).                                   */
 int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical);

@@ -42,7 +42,7 @@ EXPORTED = [
     "dqmc_set_field", "dqmc_get_field", "dqmc_set_uniforms", "dqmc_set_seed",
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
-    "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_get_greens", "dqmc_set_greens",
+    "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_get_greens", "dqmc_set_greens",
     "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
@@ -83,6 +83,7 @@ def load():
         "dqmc_reverse_build_stack": [vp], "dqmc_propagate": [vp],
         "dqmc_sweep_spatial": [vp, C.POINTER(i64)],
         "dqmc_sweep": [vp, i, C.POINTER(i64)],
+        "dqmc_batch_sweep": [C.POINTER(vp), i, i, C.POINTER(i64)],
         "dqmc_get_greens": [vp, _dp, i, i],
         "dqmc_set_greens": [vp, _dp, i],
         "dqmc_calculate_greens": [vp, i, _dp, i],
@@ -122,6 +123,20 @@ def footprint_bytes(n: int, flavors: int, slices: int, safe_mult: int, precision
     out = C.c_int64()
     _check(load().dqmc_footprint_bytes(C.byref(cfg), C.byref(out)), None, "dqmc_footprint_bytes")
     return out.value
+
+
+def batch_sweep(chains, nsweeps: int = 1) -> np.ndarray:
+    """Sweep several distinct chains concurrently (dqmc_batch_sweep); returns the accepted
+    flips of each chain."""
+    k = len(chains)
+    hs = (C.c_void_p * k)(*[c._h for c in chains])
+    acc = np.zeros(k, dtype=np.int64)
+    rc = load().dqmc_batch_sweep(hs, k, nsweeps, acc.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc != 0:
+        msgs = [load().dqmc_last_error(c._h) for c in chains]
+        raise DQMCError(f"dqmc_batch_sweep failed ({rc}): "
+                        + "; ".join(m.decode() for m in msgs if m))
+    return acc
 
 
 class HIPChain:

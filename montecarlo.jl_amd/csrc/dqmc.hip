@@ -14,6 +14,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/dqmc_hip.h"
@@ -1083,6 +1084,22 @@ int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted) {
   }
   CHK(harvest(h));
   if (accepted) { CHK(read_accepted(h, &a1)); *accepted = a1 - a0; }
+  return DQMC_OK;
+}
+
+int dqmc_batch_sweep(dqmc_chain_t* hs, int count, int nsweeps, int64_t* accepted) {
+  if (!hs || count < 1 || nsweeps < 0) return DQMC_E_INVALID;
+  for (int i = 0; i < count; i++)
+    for (int j = 0; j < i; j++)
+      if (!hs[i] || hs[i] == hs[j]) return DQMC_E_INVALID;   // a chain is not shared by two threads
+  std::vector<int> rc(count, DQMC_OK);
+  std::vector<std::thread> th;
+  th.reserve(count);
+  for (int i = 0; i < count; i++)
+    th.emplace_back([&, i]() { rc[i] = dqmc_sweep(hs[i], nsweeps, accepted ? &accepted[i] : nullptr); });
+  for (auto& t : th) t.join();
+  for (int i = 0; i < count; i++)
+    if (rc[i] != DQMC_OK) return rc[i];
   return DQMC_OK;
 }
 

@@ -179,3 +179,29 @@ def test_dqmc_driver_on_gpu_matches_ed(pkg, case):
         assert close(G[b], Ged[b * 4:(b + 1) * 4, b * 4:(b + 1) * 4])
     assert close(ms.mean("occupation"), 1 - np.diag(Ged)[: 4 * G.shape[0]])
     assert close(ms.mean("double_occupancy"), ed["double_occupancy"])
+
+
+def test_batch_sweep_matches_chains_alone(pkg, hip):
+    """Chains of different size swept together (dqmc_batch_sweep) give bit for bit what each
+    gives alone: decisions, field and Green's function."""
+    cases = [(4, 4.0, 2.0, 0.0, 11), (6, -4.0, 1.5, 0.0, 12), (8, 2.0, 1.0, -0.3, 13)]
+    def build(L, U, beta, mu, seed):
+        g, model, par, field = make_chain(hip.HIPChain, pkg, L, U, beta, mu=mu, seed=seed)
+        g.set_hopping(model.hopping_matrix())
+        g.set_field(field.conf)
+        g.set_seed(seed)
+        g.build_stack()
+        return g
+    alone = []
+    for c in cases:
+        g = build(*c)
+        acc = g.sweep(3)
+        alone.append((acc, g.get_field(), g.greens()))
+        g.close()
+    chains = [build(*c) for c in cases]
+    accs = hip.batch_sweep(chains, 3)
+    for (acc, f, G), g, a in zip(alone, chains, accs):
+        assert a == acc
+        assert np.array_equal(g.get_field(), f)
+        assert np.array_equal(g.greens(), G)
+        g.close()
