@@ -118,6 +118,15 @@ int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted);
  * update(::LocalSweep) (local_updates.jl:82) for independent chains. */
 int dqmc_batch_sweep(dqmc_chain_t* chains, int count, int nsweeps, int64_t* accepted);
 
+/* Equal-time observables on the device from the physical G (greens.jl:114-125):
+ * occupation (measurements/constructors/occupation.jl:44-55), kinetic/total energy
+ * (measurements/constructors/energy.jl:119-166), interaction energy with the reference's U
+ * (models/HubbardModel.jl:160-182), double occupancy (1 - G_up_ii)(1 - G_dn_ii).
+ * out[0] density per site, [1] kinetic, [2] interaction, [3] total energy, [4] mean double
+ * occupancy, [5, 5 + flavors n) occupation per flavour block, then n values of double
+ * occupancy per site; nout >= 5 + (flavors + 1) n.  Needs dqmc_set_hopping (T itself). */
+int dqmc_measure_equal_time(dqmc_chain_t h, double U, double* out, int nout);
+
 /* mc.stack.greens (physical = 0) or greens(mc) = e^{dtau T/2} G e^{-dtau T/2}
  * (physical = 1, greens.jl:87-125); n x (flavors*n).                                   */
 int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical);

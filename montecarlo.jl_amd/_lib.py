@@ -42,7 +42,7 @@ EXPORTED = [
     "dqmc_set_field", "dqmc_get_field", "dqmc_set_uniforms", "dqmc_set_seed",
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
-    "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_get_greens", "dqmc_set_greens",
+    "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
     "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
@@ -84,6 +84,7 @@ def load():
         "dqmc_sweep_spatial": [vp, C.POINTER(i64)],
         "dqmc_sweep": [vp, i, C.POINTER(i64)],
         "dqmc_batch_sweep": [C.POINTER(vp), i, i, C.POINTER(i64)],
+        "dqmc_measure_equal_time": [vp, d, vp, i],
         "dqmc_get_greens": [vp, _dp, i, i],
         "dqmc_set_greens": [vp, _dp, i],
         "dqmc_calculate_greens": [vp, i, _dp, i],
@@ -236,6 +237,18 @@ class HIPChain:
         a = C.c_int64()
         self._c(load().dqmc_sweep(self._h, nsweeps, C.byref(a)), "dqmc_sweep")
         return a.value
+
+    def measure_equal_time(self, U: float) -> dict:
+        """Equal-time observables computed on the device (dqmc_measure_equal_time); U is the
+        reference's coupling (positive: attractive)."""
+        n, f = self.n, self.flv
+        out = np.zeros(5 + (f + 1) * n)
+        self._c(load().dqmc_measure_equal_time(self._h, float(U), out.ctypes.data, out.size),
+                "dqmc_measure_equal_time")
+        return {"density": float(out[0]), "kinetic_energy": float(out[1]),
+                "interaction_energy": float(out[2]), "total_energy": float(out[3]),
+                "mean_double_occupancy": float(out[4]),
+                "occupation": out[5:5 + f * n].copy(), "double_occupancy": out[5 + f * n:].copy()}
 
     def _blocks(self, G):
         return np.stack([G[:, b * self.n:(b + 1) * self.n] for b in range(self.flv)])

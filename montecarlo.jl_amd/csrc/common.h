@@ -118,6 +118,9 @@ hipError_t launch_f64_to_f32(float* dst, const double* src, int64_t count, hipSt
 hipError_t launch_f32_to_f64(double* dst, const float* src, int64_t count, hipStream_t s);
 hipError_t launch_symmetrize(double* A, int n, hipStream_t s);
 // out[slot] = max |a - b| over count elements (a may be f32 when a32 != nullptr)
+// Equal-time observables from the physical G (measure.hip); out holds 5 + (flv + 1) n values.
+hipError_t launch_measure(const double* G, const double* T, int n, int flv, double U, double* out,
+                          hipStream_t s);
 hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b, int64_t count,
                              double* out, hipStream_t s);
 hipError_t launch_scale_add_identity(double* A, double scale, double add_diag, int n, hipStream_t s);

@@ -79,6 +79,7 @@ struct dqmc_chain_s {
 
   int cs = 0, cr = 1, dir = 1;
   bool hopping_set = false;
+  bool t_set = false;       // T itself is known (dqmc_set_hopping), not only its exponentials
   long long sweeps = 0, perr_count = 0, stabs = 0;
   double perr_max = 0.0, perr_last = 0.0;
   std::vector<double> stablog;
@@ -909,6 +910,7 @@ int dqmc_set_hopping(dqmc_chain_t h, const double* T, int ld) {
     norm1 = std::max(norm1, s);
   }
   CHK(upload_matrix(h, h->dT, T, ld));
+  h->t_set = true;
   CHK(expm_device(h, -0.5 * h->dtau, norm1, h->eTmh));
   CHK(expm_device(h, +0.5 * h->dtau, norm1, h->eTph));
   CHK(expm_device(h, -h->dtau, norm1, h->eT2m));
@@ -1103,9 +1105,9 @@ int dqmc_batch_sweep(dqmc_chain_t* hs, int count, int nsweeps, int64_t* accepted
   return DQMC_OK;
 }
 
-int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical) {
-  ENTER(h);
-  if (!G || ld < h->n) { h->err = "dqmc_get_greens: null G or ld < n"; return DQMC_E_INVALID; }
+// Effective (physical = 0) or physical G on the device: greens.jl:114-125,
+// G_phys = e^{+dtau T/2} G e^{-dtau T/2}.  The physical form is written to O (Gx is scratch).
+int device_greens(Chain* h, int physical, const double** out) {
   const double* src = h->G;
   if (h->prec == DQMC_PRECISION_MIXED) {
     HIPCHK(h, launch_f32_to_f64(h->Gchk, h->G32, (int64_t)h->flv * h->nn, h->st));
@@ -1122,7 +1124,31 @@ int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical) {
     CHK(run(h, g2));
     src = h->O;
   }
+  *out = src;
+  return DQMC_OK;
+}
+
+int dqmc_get_greens(dqmc_chain_t h, double* G, int ld, int physical) {
+  ENTER(h);
+  if (!G || ld < h->n) { h->err = "dqmc_get_greens: null G or ld < n"; return DQMC_E_INVALID; }
+  const double* src = nullptr;
+  CHK(device_greens(h, physical, &src));
   CHK(download_matrix(h, G, ld, src, h->flv * h->n));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return DQMC_OK;
+}
+
+int dqmc_measure_equal_time(dqmc_chain_t h, double U, double* out, int nout) {
+  ENTER(h);
+  NEED_HOPPING(h);
+  const int need = 5 + (h->flv + 1) * h->n;
+  if (!out || nout < need) { h->err = "dqmc_measure_equal_time: out needs 5 + (flavors + 1) n values"; return DQMC_E_INVALID; }
+  if (!h->t_set) { h->err = "dqmc_measure_equal_time: needs T (dqmc_set_hopping), not only its exponentials"; return DQMC_E_STATE; }
+  const double* G = nullptr;
+  CHK(device_greens(h, 1, &G));
+  double* dout = h->Gx;   // free after the physical transform
+  HIPCHK(h, launch_measure(G, h->dT, h->n, h->flv, U, dout, h->st));
+  HIPCHK(h, hipMemcpyAsync(out, dout, sizeof(double) * need, hipMemcpyDeviceToHost, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return DQMC_OK;
 }

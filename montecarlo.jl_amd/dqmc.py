@@ -82,7 +82,8 @@ class LocalSweep:
 class DQMC:
     def __init__(self, model, *, seed: int = -1, thermalization: int = 100, sweeps: int = 100,
                  measure_rate: int = 10, last_sweep: int = 0, field=None, precision: str = "fp64",
-                 device: int = 0, chain_factory=None, silent: bool = False, **kwargs):
+                 device: int = 0, chain_factory=None, silent: bool = False,
+                 device_measurements: bool = False, **kwargs):
         self.parameters = DQMCParameters.create(thermalization=thermalization, sweeps=sweeps,
                                                 measure_rate=measure_rate, silent=silent, **kwargs)
         self.model = model
@@ -98,6 +99,9 @@ class DQMC:
         self.measurements = {}
         self.thermalization_measurements = {}
         self._chain_factory = chain_factory
+        # equal-time observables computed on the device (G stays in HBM); the mean Green's
+        # function is then not recorded
+        self.device_measurements = device_measurements
         self.stack = None
 
     # ---- stack (drop-in for DQMCStack) ------------------------------------------------
@@ -156,7 +160,10 @@ class DQMC:
             self.acceptance.total += 1
             max_acc = max(max_acc, rate)
             if self.last_sweep > p.thermalization and self.last_sweep % p.measure_rate == 0:
-                self.measurements["equal_time"].push(self.greens(physical=True))
+                if self.device_measurements and hasattr(self.stack, "measure_equal_time"):
+                    self.measurements["equal_time"].push_values(self.stack.measure_equal_time(self.model.U))
+                else:
+                    self.measurements["equal_time"].push(self.greens(physical=True))
             if self.last_sweep == p.thermalization:
                 self.analysis.th_runtime += time.perf_counter() - t0
                 t0 = time.perf_counter()

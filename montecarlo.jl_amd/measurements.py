@@ -75,5 +75,13 @@ class MeasurementSet:
             self.sums[k] = self.sums.get(k, 0.0) + v
         self.count += 1
 
+    def push_values(self, vals: dict):
+        """One sample of already computed observables (the device path: dqmc_measure_equal_time)."""
+        keys = ("occupation", "double_occupancy", "density", "kinetic_energy", "interaction_energy",
+                "total_energy")
+        for k in keys:
+            self.sums[k] = self.sums.get(k, 0.0) + vals[k]
+        self.count += 1
+
     def mean(self, key: str):
         return self.sums[key] / max(self.count, 1)

@@ -205,3 +205,42 @@ def test_batch_sweep_matches_chains_alone(pkg, hip):
         assert np.array_equal(g.get_field(), f)
         assert np.array_equal(g.greens(), G)
         g.close()
+
+
+@pytest.mark.parametrize("L,U,beta,mu", [(4, 4.0, 2.0, 0.0), (4, -4.0, 2.0, 0.0), (6, 2.0, 1.5, -0.4)])
+def test_device_measurements_match_host(pkg, hip, L, U, beta, mu):
+    """dqmc_measure_equal_time against the host kernels (measurements.py) on the same physical G."""
+    from montecarlo_jl_amd import measurements as ms
+    g, model, par, field = make_chain(hip.HIPChain, pkg, L, U, beta, mu=mu, seed=5)
+    T = model.hopping_matrix()
+    g.set_hopping(T)
+    g.set_field(field.conf)
+    g.set_seed(3)
+    g.build_stack()
+    g.sweep(1)
+    G = g.greens(physical=True)
+    d = g.measure_equal_time(U)
+    tol = lambda a, b: abs(a - b) <= 1e-12 * max(1.0, abs(b))
+    assert tol(d["density"], ms.density(G))
+    assert tol(d["kinetic_energy"], ms.kinetic_energy(G, T))
+    assert tol(d["interaction_energy"], ms.interaction_energy(G, U))
+    assert tol(d["total_energy"], ms.total_energy(G, T, U))
+    assert np.allclose(d["occupation"], ms.occupation(G), rtol=0, atol=1e-13)
+    assert np.allclose(d["double_occupancy"], ms.double_occupancy(G), rtol=0, atol=1e-13)
+    g.close()
+
+
+def test_driver_device_measurements_match_host_path(pkg):
+    """The run! loop with observables measured on the device equals the host path sample for
+    sample (same seed, same chain)."""
+    model = pkg.HubbardModel(4, 2, U=-4.0, mu=0.0, t=1.0)
+    kw = dict(beta=2.0, delta_tau=0.1, safe_mult=10, thermalization=5, sweeps=20, measure_rate=2, seed=9)
+    a = pkg.DQMC(model, **kw)
+    b = pkg.DQMC(model, device_measurements=True, **kw)
+    assert a.run(verbose=False) == pkg.ExitCode.SUCCESS
+    assert b.run(verbose=False) == pkg.ExitCode.SUCCESS
+    ma, mb = a.measurements["equal_time"], b.measurements["equal_time"]
+    assert ma.count == mb.count > 0
+    for k in ("density", "kinetic_energy", "interaction_energy", "total_energy"):
+        assert abs(ma.mean(k) - mb.mean(k)) <= 1e-11 * max(1.0, abs(ma.mean(k)))
+    assert np.allclose(ma.mean("double_occupancy"), mb.mean("double_occupancy"), rtol=0, atol=1e-12)
