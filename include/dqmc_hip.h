@@ -49,7 +49,8 @@ typedef struct {
   double alpha;     /* Hirsch coupling field.alpha (fields.jl:370-376, 419-425)             */
   int precision;    /* DQMC_PRECISION_*                                                     */
   int check_propagation_error; /* DQMCParameters.check_propagation_error                   */
-  int delay_depth;  /* sites per local-update window: <= 32 -> 32, else 64; 0 = automatic     */
+  int delay_depth;  /* sites per local-update window: <= 16 -> 16, <= 32 -> 32, else 64;
+                       0 = automatic (32)                                              */
   int device;       /* HIP device ordinal                                                   */
 } dqmc_config_t;
 

@@ -806,7 +806,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   h->bweight[1] = std::exp(-(-2.0 * a * 1.0));
 
   h->win = window_size(h->n, h->flv);
-  if (cfg->delay_depth > 0) h->win = cfg->delay_depth <= 32 ? 32 : 64;
+  if (cfg->delay_depth > 0) h->win = cfg->delay_depth <= 16 ? 16 : cfg->delay_depth <= 32 ? 32 : 64;
 
   const size_t f = h->flv, nn = h->nn, n = h->n;
   int r = DQMC_OK;

@@ -489,8 +489,9 @@ static void launch_apply_w(GT* Gd, const GT* Gs, const int* S, const GT* Y, int 
 template <typename GT>
 static hipError_t launch_apply_t(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
                                  hipStream_t s) {
-  if (W != 32 && W != 64) return hipErrorInvalidValue;
-  if (W == 32) launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s);
+  if (W != 16 && W != 32 && W != 64) return hipErrorInvalidValue;
+  if (W == 16) launch_apply_w<GT, 16>(Gd, Gs, S, Y, n, W, flv, s);
+  else if (W == 32) launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s);
   else launch_apply_w<GT, 64>(Gd, Gs, S, Y, n, W, flv, s);
   return hipGetLastError();
 }
@@ -507,9 +508,12 @@ size_t window_lds_bytes(int W, int flv) {
   return (size_t)2 * flv * W * (W + 1) * 8 + (size_t)flv * W * 8 + (size_t)W * 4;
 }
 
+// 32 sites per window: measured fastest at n = 256 (per-flip work and the m x m solves
+// grow with W, launches with 1 / W); 16 and 64 are selectable through delay_depth.
 int window_size(int n, int flv) {
+  (void)n;
   (void)flv;
-  return n <= 32 ? 32 : 64;
+  return 32;
 }
 
 template <typename GT, int FLV, int W>
@@ -519,11 +523,15 @@ static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
 
 template <typename GT>
 static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
-  if (a.flv < 1 || a.flv > 2 || (a.W != 32 && a.W != 64)) return hipErrorInvalidValue;
+  if (a.flv < 1 || a.flv > 2 || (a.W != 16 && a.W != 32 && a.W != 64)) return hipErrorInvalidValue;
   if (a.flv == 1) {
-    if (a.W == 32) launch_window_w<GT, 1, 32>(a, s); else launch_window_w<GT, 1, 64>(a, s);
+    if (a.W == 16) launch_window_w<GT, 1, 16>(a, s);
+    else if (a.W == 32) launch_window_w<GT, 1, 32>(a, s);
+    else launch_window_w<GT, 1, 64>(a, s);
   } else {
-    if (a.W == 32) launch_window_w<GT, 2, 32>(a, s); else launch_window_w<GT, 2, 64>(a, s);
+    if (a.W == 16) launch_window_w<GT, 2, 16>(a, s);
+    else if (a.W == 32) launch_window_w<GT, 2, 32>(a, s);
+    else launch_window_w<GT, 2, 64>(a, s);
   }
   return hipGetLastError();
 }
@@ -531,7 +539,13 @@ hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s) { retur
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s) { return launch_window_t(a, s); }
 
 hipError_t local_kernels_init() {
-  const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double, 1, 32>),
+  const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double, 1, 16>),
+                       reinterpret_cast<const void*>(window_kernel<double, 2, 16>),
+                       reinterpret_cast<const void*>(window_kernel<float, 1, 16>),
+                       reinterpret_cast<const void*>(window_kernel<float, 2, 16>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 16>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
+                       reinterpret_cast<const void*>(window_kernel<double, 1, 32>),
                        reinterpret_cast<const void*>(window_kernel<double, 1, 64>),
                        reinterpret_cast<const void*>(window_kernel<double, 2, 32>),
                        reinterpret_cast<const void*>(window_kernel<double, 2, 64>),

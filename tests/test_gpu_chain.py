@@ -111,6 +111,41 @@ def test_one_sweep_accept_sequence(pkg, hip, oracle, L, U, beta, mu, stream):
     assert g.state() == o.state() == (1, 1, 1)
 
 
+@pytest.mark.parametrize("L,U,beta,mu,lattice,window", [
+    (10, -4.0, 1.0, 0.0, None, 0),      # n = 100: windows 32+32+32+4
+    (10, 4.0, 1.0, -0.3, None, 16),     # 16-site windows
+    (10, -3.0, 1.0, 0.0, None, 64),     # 64-site windows: 64 + 36
+    (4, 3.0, 1.0, 0.1, "honeycomb", 0),  # honeycomb 2 L^2 = 32 sites
+])
+def test_window_widths_and_lattices(pkg, hip, oracle, L, U, beta, mu, lattice, window):
+    """One sweep with supplied uniforms: decisions, field and G against the oracle for window
+    widths 16 / 32 / 64, a lattice whose size is no multiple of the window, and honeycomb."""
+    lat = pkg.Honeycomb(L) if lattice == "honeycomb" else None
+    g, model, par, field = make_chain(hip.HIPChain, pkg, L, U, beta, mu=mu, lattice=lat, seed=21,
+                                      delay_depth=window)
+    o, *_ = make_chain(oracle.OracleChain, pkg, L, U, beta, mu=mu, lattice=lat, seed=21)
+    o.set_hopping(model.hopping_matrix())
+    g.set_hopping_exp(*o.get_hopping_exp())
+    g.set_field(field.conf)
+    o.set_field(field.conf)
+    n, M = len(model.l), par.slices
+    u = np.random.default_rng(7).random(2 * n * M)
+    g.set_uniforms(u)
+    o.set_uniforms(u)
+    g.set_log(2 * n * M)
+    o.set_log(2 * n * M)
+    g.build_stack()
+    o.build_stack()
+    g.sweep(1)
+    o.sweep(1)
+    dg, _ = g.get_log()
+    do, _ = o.get_log()
+    if g.counters()["ties"] == 0 and o.counters()["ties"] == 0:
+        assert np.array_equal(dg, do)
+        assert np.array_equal(g.get_field(), o.get_field())
+    assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+
+
 def test_u0_closed_form(pkg, hip):
     """U = 0: G = [I + exp(-beta T)]^-1 at 1e-12 (flavortests_DQMC.jl:355-386)."""
     for model in (pkg.HubbardModel(7, 2, U=0.0, t=1.0), pkg.HubbardModel(8, 2, U=0.0, mu=1.0, t=1.0)):
