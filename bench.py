@@ -117,20 +117,28 @@ def kernel_work(kind, n, flv, win, per_launch_accepted=0.0):
     if kind == "udt_panel":
         pans = udt_panels(n)
         return flv * sum(2.0 * (n - p0) * w * w for p0, w in pans) / len(pans)
+    nw = -(-n // win)                 # windows per slice; the last one's update runs alone
     if kind == "local_apply":
         m = per_launch_accepted
         return 2.0 * flv * (n * n * m + m * m * n)
     if kind == "local":
+        # window decisions, plus the previous window's update in all but the first launch
         m = per_launch_accepted
-        return flv * (2.0 * win * win * m + (2.0 / 3.0) * m ** 3)
+        own = flv * (2.0 * win * win * m + (2.0 / 3.0) * m ** 3)
+        return own + (nw - 1) / nw * 2.0 * flv * (n * n * m + m * m * n)
     return 0.0
 
 
 def kernel_bytes(kind, n, flv, win):
-    """Algorithmic HBM bytes per launch of the window kernel (the latency/memory-bound kernel):
-    the window block and its diagonal read, Y written, field column, flip list and log."""
+    """Algorithmic HBM bytes per launch of the window kernel: the window block, the strips
+    G[w, wp], G[wp, w] and the previous factor read, the factor written, field column and
+    log; plus, in all but the first launch of a slice, the previous window's update of G
+    (G read once, written once)."""
     if kind == "local":
-        return 2.0 * flv * win * win * 8 + flv * win * 8 + 3 * win + 4 * (win + 1)
+        nw = -(-n // win)
+        own = 4.0 * flv * win * win * 8 + flv * win * win * 8 + 3 * win + 4
+        upd = 2.0 * flv * n * n * 8
+        return own + (nw - 1) / nw * upd
     return 0.0
 
 
@@ -214,7 +222,7 @@ def main():
     acc_k = ch.stats()["accepted"] - acc_k0
     ch.set_kernel_timing(False)
     kernels = {}
-    win = 32 if n <= 32 else 64
+    win = 32                          # the library's default window (delay_depth 0)
     windows = kt["local"][1]
     for kind, (ms, cnt) in kt.items():
         if cnt == 0:

@@ -147,18 +147,27 @@ struct ChainCounters {
   long long flushes;
 };
 
-// Window kernel of the local update (local.hip): sites [w0, w0 + W) of one slice.
+// Window kernel of the local update (local.hip): sites [w0, w0 + W) of one slice.  The
+// same launch applies the previous window's factors to all of G (copy G -> Gd), and the
+// window's own block is formed from G and those factors, so consecutive windows need one
+// launch each and the application is off the decisions' critical path.
 template <typename GT>
 struct WindowArgs {
-  const GT* G;            // flv blocks, n x n, ld = n (read only here)
+  const GT* G;            // flv blocks, n x n, ld = n: G before the previous window's flips
+  GT* Gd;                 // out: G after the previous window's flips (when Yp != nullptr)
   int n, flv;
   int8_t* conf;           // conf column of the current slice
-  int w0, W;              // W <= 64
+  int w0, W;              // W in {16, 32, 64}
+  int wp;                 // start of the previous window (w0 - W)
+  int do_window;          // 0: only the application of the previous window
   double d_first[2];      // Delta block 1, index (x > 0)
   double d_second[2];     // Delta block 2 (magnetic)
   double bweight[2];      // exp(-dE_boson) density, index (x > 0)
-  int* S;                 // out: S[0] = m accepted, S[1 + k] = site of flip k (0-based)
-  GT* Y;                  // out: flv x (W x W), zero beyond m
+  const int* Sp;          // previous window: Sp[0] = flips (nullptr: no previous window)
+  const GT* Yp;           // previous window: dense factor, flv x (W x W)
+  int* S;                 // out: S[0] = m accepted flips of this window
+  GT* Y;                  // out: dense factor over window positions, flv x (W x W) row-major,
+                          //      zero in the rows and columns of rejected sites
   const double* uniforms; // supplied stream (nullptr: counter stream)
   long long n_uniforms;
   unsigned long long seed;
@@ -171,10 +180,5 @@ hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
 int window_size(int n, int flv);
 size_t window_lds_bytes(int W, int flv);
-// Gd_b = Gs_b - C_b Y_b R_b for every block b, C = (I - Gs)[:, S], R = Gs[S, :]
-hipError_t launch_window_apply_f64(double* Gd, const double* Gs, const int* S, const double* Y, int n, int W,
-                                   int flv, hipStream_t s);
-hipError_t launch_window_apply_f32(float* Gd, const float* Gs, const int* S, const float* Y, int n, int W,
-                                   int flv, hipStream_t s);
 
 }  // namespace dqmc

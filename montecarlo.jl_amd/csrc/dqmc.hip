@@ -63,8 +63,8 @@ struct dqmc_chain_s {
   float *G32 = nullptr, *Gx32 = nullptr, *Gchk32 = nullptr, *eT2m32 = nullptr, *eT2p32 = nullptr;
   double* Gpp = nullptr;   // second copy of G for the window updates (FP64 form)
   float* G32pp = nullptr;  // second copy of G32 (mixed form)
-  double* wY = nullptr;    // window factor Y (flv x W x W, FP64 or FP32 contents)
-  int* wS = nullptr;       // window flips: count and sites
+  double* wY = nullptr;    // window factors Y, two slots of flv x W x W (FP64 or FP32 contents)
+  int* wS = nullptr;       // window flip counts, two slots
   int win = 32;
   ChainCounters* ctr = nullptr;
   double* unif = nullptr;
@@ -646,35 +646,48 @@ int forward_build(Chain* h) {
 // the other); after an odd number of windows the copies swap roles.
 template <typename GT>
 int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
+  // Window k's launch also applies window k - 1 (source copy -> other copy); the factors
+  // of consecutive windows alternate between two slots.
   const int n = h->n, W = h->win;
+  const size_t ysz = (size_t)h->flv * W * W;
   GT* src = *Gp;
   GT* dst = *Gpp;
+  WindowArgs<GT> a;
+  a.n = n; a.flv = h->flv;
+  a.conf = h->conf + (size_t)(h->cs - 1) * n;
+  a.W = W;
+  for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
+  a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
+  a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
+  int slot = 0;
+  auto launch = [&](int kind) -> int {
+    Timed t(h, kind);
+    hipError_t e;
+    if constexpr (sizeof(GT) == 8) e = launch_window_f64(a, h->st);
+    else e = launch_window_f32(a, h->st);
+    HIPCHK(h, e);
+    return DQMC_OK;
+  };
   for (int w0 = 0; w0 < n; w0 += W) {
-    WindowArgs<GT> a;
-    a.G = src;
-    a.n = n; a.flv = h->flv;
-    a.conf = h->conf + (size_t)(h->cs - 1) * n;
-    a.w0 = w0; a.W = W;
-    for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
-    a.S = h->wS; a.Y = Y;
-    a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
-    a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
-    {
-      Timed t(h, DQMC_KERNEL_LOCAL);
-      hipError_t e;
-      if constexpr (sizeof(GT) == 8) e = launch_window_f64(a, h->st);
-      else e = launch_window_f32(a, h->st);
-      HIPCHK(h, e);
-    }
-    {
-      Timed t(h, DQMC_KERNEL_LOCAL_APPLY);
-      hipError_t e;
-      if constexpr (sizeof(GT) == 8) e = launch_window_apply_f64(dst, src, h->wS, Y, n, W, h->flv, h->st);
-      else e = launch_window_apply_f32(dst, src, h->wS, Y, n, W, h->flv, h->st);
-      HIPCHK(h, e);
-    }
-    std::swap(src, dst);
+    const bool prev = w0 > 0;
+    a.G = src; a.Gd = dst;
+    a.w0 = w0; a.wp = w0 - W; a.do_window = 1;
+    a.Sp = prev ? h->wS + (slot ^ 1) * (W + 1) : nullptr;
+    a.Yp = prev ? Y + (slot ^ 1) * ysz : nullptr;
+    a.S = h->wS + slot * (W + 1);
+    a.Y = Y + slot * ysz;
+    CHK(launch(DQMC_KERNEL_LOCAL));
+    if (prev) std::swap(src, dst);
+    slot ^= 1;
   }
+  // the last window's update
+  a.G = src; a.Gd = dst;
+  a.wp = ((n - 1) / W) * W; a.w0 = a.wp; a.do_window = 0;
+  a.Sp = h->wS + (slot ^ 1) * (W + 1);
+  a.Yp = Y + (slot ^ 1) * ysz;
+  a.S = nullptr; a.Y = nullptr;
+  CHK(launch(DQMC_KERNEL_LOCAL_APPLY));
+  std::swap(src, dst);
   *Gp = src;
   *Gpp = dst;
   return DQMC_OK;
@@ -748,7 +761,7 @@ int64_t footprint(const dqmc_config_t* c, int* Kout) {
   if (c->precision == DQMC_PRECISION_MIXED) b += (3 * f * nn + 2 * nn) * 4;
   const int64_t W = window_size((int)n, (int)f);
   const int64_t welem = c->precision == DQMC_PRECISION_MIXED ? 4 : 8;
-  b += (f * nn + f * W * W) * welem + (W + 1) * 4;   // second copy of G, window factors Y, S
+  b += (f * nn + 2 * f * W * W) * welem + 2 * (W + 1) * 4; // second copy of G, window factors Y, S (two slots)
   b += 4096 * 8 + 4096;                              // error slots, counters
   return b;
 }
@@ -828,7 +841,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   } else {
     A(&h->Gpp, f * nn);
   }
-  A(&h->wY, f * h->win * h->win); A(&h->wS, (size_t)h->win + 1);
+  A(&h->wY, 2 * f * h->win * h->win); A(&h->wS, 2 * ((size_t)h->win + 1));
   A(&h->ctr, 1);
   A(&h->errbuf, (size_t)h->errcap);
   A(&h->scal, 16);

@@ -5,12 +5,15 @@
 // accept_local! / update_greens! (fields.jl:271-286, 340-344) with the rank-1 helpers
 // of linalg/updates.jl:7-131.
 //
-// Design: the sites of a slice are visited in windows of W <= 64 consecutive sites.
-//  * window_kernel (one workgroup per chain): Metropolis decisions and rank-1 updates on
-//    the W x W block of G held in registers, and the factors S, Y of the accumulated
-//    update G <- G - C Y R with C = (I - G)[:, S], R = G[S, :].
-//  * window_apply_kernel (one 64 x 64 tile per workgroup): the rank-m update of all of G
-//    on the matrix cores, from one copy of G into the other.
+// Design: the sites of a slice are visited in windows of W in {16, 32, 64} consecutive
+// sites, one launch per window:
+//  * workgroup 0 forms the window's W x W block of G (applying the previous window's
+//    factors to it), runs the Metropolis decisions and rank-1 updates on the block held in
+//    registers, and writes the dense factor Y of the accumulated update
+//    G <- G - C Y R with C = (I - G)[:, w], R = G[w, :] over the window's positions w;
+//  * workgroups 1.. apply the previous window's update to all of G on the matrix cores,
+//    one 64 x 64 tile each, from one copy of G into the other;
+//  * window_apply_kernel applies the last window's update.
 // This is algebraically the reference's sequence of rank-1 updates; only the rounding
 // differs, so accept decisions agree except at ties |u - p| ~ 1e-12, which are counted.
 #include "common.h"
@@ -105,11 +108,100 @@ constexpr size_t window_lds() {
   return (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4;
 }
 
+// The window's block of G after the previous window's flips, from the source copy:
+//   B = G[w, w] - C[w, :] Y R[:, w] = G[w, w] + G[w, wp] (Y G[wp, w])
+// (C = (I - G)[:, wp] has no identity part in the rows of this window), on the matrix
+// cores with the operands, order and roles of the application's tiles, so that B equals
+// the block the application writes.  Wave unit (b, ci) forms the 16-column tile ci of
+// flavour block b; the result goes to LDS as Bn[b][c][r] (leading dimension W + 1).
 template <typename GT, int FLV, int W>
-__global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
+__device__ __forceinline__ void window_block(const WindowArgs<GT>& a, double* Bn, GT* Zs, int Wn, int lane,
+                                             int wave) {
+  typedef typename V4<GT>::type acc_t;
+  constexpr int CT = W / 16, KS = W / 4, LD = W + 1;
+  const int n = a.n, w0 = a.w0, wp = a.wp;
+  const size_t nn = (size_t)n * n;
+  const int li = lane & 15, lk = lane >> 4;
+  const bool corr = a.Yp != nullptr;
+  const int mp = corr ? a.Sp[0] : 0;
+  for (int unit = wave; unit < FLV * CT; unit += 4) {
+    const int b = unit / CT, ci = unit - b * CT;
+    const GT* Gb = a.G + (size_t)b * nn;
+    const int cc = ci * 16 + li;
+    const size_t colo = (size_t)(w0 + (cc < Wn ? cc : Wn - 1)) * n;
+    acc_t o[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = t * 16 + mfma_row<GT>(lane, r);
+        o[t][r] = Gb[(size_t)(w0 + (row < Wn ? row : Wn - 1)) + colo];
+      }
+    if (corr) {
+      const GT* Yb = a.Yp + (size_t)b * W * W;
+      GT qb[KS], ya[CT][KS], pa[CT][KS];
+#pragma unroll
+      for (int s = 0; s < KS; s++) {
+        const int k = wp + 4 * s + lk;
+        qb[s] = Gb[(size_t)(k < n ? k : n - 1) + colo];
+      }
+#pragma unroll
+      for (int t = 0; t < CT; t++) {
+        const int rr = t * 16 + li;
+        const size_t rowo = (size_t)(w0 + (rr < Wn ? rr : Wn - 1));
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+          const int k = wp + 4 * s + lk;
+          ya[t][s] = Yb[(size_t)(t * 16 + li) * W + 4 * s + lk];
+          pa[t][s] = Gb[rowo + (size_t)(k < n ? k : n - 1) * n];
+        }
+      }
+      if (mp > 0) {
+        acc_t z[CT];
+#pragma unroll
+        for (int t = 0; t < CT; t++) {
+          const acc_t zero = {0, 0, 0, 0};
+          z[t] = zero;
+#pragma unroll
+          for (int s = 0; s < KS; s++) z[t] = mfma16(ya[t][s], qb[s], z[t]);   // Z = Y G[wp, w]
+        }
+        if constexpr (sizeof(GT) == 8) {
+          // element r of Z tile t is Z[16 t + lk + 4 r][cc]: the B operand of k-step 4 t + r
+#pragma unroll
+          for (int t = 0; t < CT; t++)
+#pragma unroll
+            for (int s = 0; s < KS; s++) o[t] = mfma16(pa[t][s], z[s >> 2][s & 3], o[t]);
+        } else {
+          // FP32 accumulator rows are 4 (lane / 16) + r: regroup Z through LDS (wave-local)
+          GT* Zb = Zs + (size_t)b * W * LD;
+#pragma unroll
+          for (int t = 0; t < CT; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) Zb[(t * 16 + mfma_row<GT>(lane, r)) * LD + cc] = z[t][r];
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+          for (int s = 0; s < KS; s++) {
+            const GT bz = Zb[(4 * s + lk) * LD + cc];
+#pragma unroll
+            for (int t = 0; t < CT; t++) o[t] = mfma16(pa[t][s], bz, o[t]);
+          }
+        }
+      }
+    }
+    double* Bb = Bn + (size_t)b * W * LD;
+#pragma unroll
+    for (int t = 0; t < CT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) Bb[cc * LD + t * 16 + mfma_row<GT>(lane, r)] = (double)o[t][r];
+  }
+}
+
+template <typename GT, int FLV, int W>
+__device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned char* smem) {
   constexpr int NW = 4, CPW = W / NW, LD = W + 1;
   typedef double colvec __attribute__((ext_vector_type(CPW)));
-  extern __shared__ __align__(16) unsigned char smem[];
   // LDS, per flavour block b: Uw[b][l][r] = u_l(r);  Vw[b][l][vpos(c)] = G_l[s_l][c] (the
   // raw row; v_l = gamma_l times it)
   double* Uw = reinterpret_cast<double*>(smem);
@@ -119,29 +211,11 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
   const int n = a.n, w0 = a.w0;
   const int Wn = n - w0 < W ? n - w0 : W;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const GT* __restrict__ G = a.G;
-  const size_t nn = (size_t)n * n;
-
   const long long pos0 = a.ctr->u_pos;
   const long long log0 = a.ctr->log_len;
-  // window block (registers), diagonal copy, field, this window's uniforms (lane k: u_{pos0+k})
   const int lr = lane < Wn ? lane : Wn - 1;
   const bool rowok = lane < Wn;
-  colvec g0, g1;
-#pragma unroll
-  for (int i = 0; i < CPW; i++) {
-    const int c = wave + NW * i;
-    const int cc = c < Wn ? c : Wn - 1;
-    const size_t o = (size_t)(w0 + lr) + (size_t)(w0 + cc) * n;
-    const double x0 = (double)G[o];
-    const double x1 = FLV > 1 ? (double)G[nn + o] : 0.0;
-    const bool ok = rowok && c < Wn;
-    g0[i] = ok ? x0 : 0.0;
-    g1[i] = ok ? x1 : 0.0;
-  }
-  const size_t od = (size_t)(w0 + lr) * (n + 1);
-  double d0 = rowok ? (double)G[od] : 0.0;
-  double d1 = (FLV > 1 && rowok) ? (double)G[nn + od] : 0.0;
+  // field and this window's uniforms (lane k: u_{pos0 + k})
   const int cfl = (int)a.conf[w0 + lr];
   double ul = 1.0;
   if (a.uniforms) {
@@ -149,6 +223,25 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
   } else {
     ul = counter_uniform(a.seed, (unsigned long long)(pos0 + lane));
   }
+  // window block after the previous window's flips (LDS, aliasing Uw), then into registers:
+  // lane = row, wave w keeps columns w, w + 4, ...; every wave keeps the diagonal (lane p: G_pp)
+  double* Bn = Uw;
+  window_block<GT, FLV, W>(a, Bn, reinterpret_cast<GT*>(Vw), Wn, lane, wave);
+  __syncthreads();
+  colvec g0, g1;
+#pragma unroll
+  for (int i = 0; i < CPW; i++) {
+    const int c = wave + NW * i;
+    const int cc = c < Wn ? c : Wn - 1;
+    const double x0 = Bn[cc * LD + lr];
+    const double x1 = FLV > 1 ? Bn[(W + cc) * LD + lr] : 0.0;
+    const bool ok = rowok && c < Wn;
+    g0[i] = ok ? x0 : 0.0;
+    g1[i] = ok ? x1 : 0.0;
+  }
+  double d0 = rowok ? Bn[lr * LD + lr] : 0.0;
+  double d1 = (FLV > 1 && rowok) ? Bn[(W + lr) * LD + lr] : 0.0;
+  __syncthreads();                  // Bn is overwritten by the first flip's u, v
 
   const double df0 = a.d_first[0], df1 = a.d_first[1];
   const double ds0 = a.d_second[0], ds1 = a.d_second[1];
@@ -304,17 +397,26 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
         xr[j] = fma(gb[j], (s4[0] + s4[1]) + (s4[2] + s4[3]), xr[j]);
       }
     }
-    // Y is stored row-major (Y[j][c] at j * W + c) so that lanes write consecutive columns
+    // dense factor over window positions: Y[b][s_j][s_l] = y_j(l) (lane l holds column l)
     GT* Yb = a.Y + (size_t)b * W * W;
-    if (lane < W) {
+    if (lane < m) {
 #pragma unroll
-      for (int j = 0; j < W; j++) Yb[(size_t)j * W + lane] = (GT)xr[j];
+      for (int j = 0; j < W; j++)
+        if (j < m) Yb[(size_t)sites[j] * W + site_r] = (GT)xr[j];
     }
   }
-  if (wave == 0) {
-    if (lane < m) a.S[1 + lane] = w0 + site_r;
-    if (lane == 0) a.S[0] = m;
+  if (m > 0) {
+    // zeros in the rows and columns of rejected sites (disjoint from the entries above)
+    const unsigned long long accm = __ballot(acc_r != 0);
+#pragma unroll
+    for (int q = 0; q < FLV * W * W / 256; q++) {
+      const int e = tid + 256 * q;
+      const int rem = e % (W * W);
+      const int pr = rem / W, pc = rem - pr * W;
+      if (!(((accm >> pr) & 1ull) && ((accm >> pc) & 1ull))) a.Y[e] = GT(0);
+    }
   }
+  if (tid == 0) a.S[0] = m;
   if (tid == 0) {
     ChainCounters* c = a.ctr;
     c->u_pos = pos0 + used;
@@ -328,36 +430,22 @@ __global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
   }
 }
 
-// Load K elements per thread with every load in flight before the first LDS store.
-template <int K, typename T, typename Ld, typename St>
-__device__ __forceinline__ void stage(int total, int tid, Ld ld, St st) {
-  T v[K];
-#pragma unroll
-  for (int q = 0; q < K; q++) {
-    const int idx = tid + 256 * q;
-    v[q] = idx < total ? ld(idx) : T(0);
-  }
-#pragma unroll
-  for (int q = 0; q < K; q++) {
-    const int idx = tid + 256 * q;
-    if (idx < total) st(idx, v[q]);
-  }
-}
-
-// Gd_b = Gs_b - C_b (Y_b R_b) for one 64 x 64 tile per workgroup, with C = (I - Gs)[:, S]
-// and R = Gs[S, :] gathered from the source copy (the update writes the other copy, so
-// no tile can change a column or row that another workgroup still gathers).  The tile's
-// slice of Z = Y R (W16 x 64) is formed in LDS on the matrix cores; the rank-W16 product
-// is computed transposed, tile^T -= Z^T C^T, so accumulator element r of lane l is
+// Gd_b = Gs_b - C_b (Y_b R_b) on one 64 x 64 tile, for the previous window's flips with
+// its dense factor Y over positions wp .. wp + W - 1: C = (I - Gs)[:, wp:wp+W] and
+// R = Gs[wp:wp+W, :] are contiguous strips of the source copy (the update writes the other
+// copy, so no tile changes what another still reads).  The tile's slice of Z = Y R
+// (W x 64) is formed in LDS on the matrix cores; the rank-W product is computed
+// transposed, tile^T -= Z^T C^T, so accumulator element r of lane l is
 // G[r0 + l%16][c0 + row(l, r)]: quarter-waves touch 16 consecutive rows of a column.
-template <typename GT, int W16>
-__global__ __launch_bounds__(256) void window_apply_kernel(GT* __restrict__ Gd, const GT* __restrict__ Gs,
-                                                           const int* __restrict__ S,
-                                                           const GT* __restrict__ Y, int n, int W) {
+template <typename GT, int W>
+__device__ __forceinline__ void apply_tile(GT* __restrict__ Gd, const GT* __restrict__ Gs,
+                                          const int* __restrict__ Sp, const GT* __restrict__ Y, int n,
+                                          int wp, int tile, unsigned char* smem) {
   typedef typename V4<GT>::type acc_t;
-  extern __shared__ __align__(16) unsigned char smem[];
-  const int b = blockIdx.z;
-  const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+  const int nt = (n + 63) >> 6;
+  const int b = tile / (nt * nt);
+  const int rem = tile - b * nt * nt;
+  const int r0 = (rem % nt) * 64, c0 = (rem / nt) * 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const size_t off = (size_t)b * n * n;
@@ -374,93 +462,79 @@ __global__ __launch_bounds__(256) void window_apply_kernel(GT* __restrict__ Gd, 
       const int gcol = c0 + 16 * j + mfma_row<GT>(lane, r);
       acc[j][r] = Gb[(size_t)growc + (size_t)(gcol < n ? gcol : n - 1) * n];
     }
-  const int m = S[0];
+  const int m = Sp[0];
   if (m > 0) {
-    const int m16 = (m + 15) & ~15;
-    GT* Ys = reinterpret_cast<GT*>(smem);   // W16 x W16 column-major: Ys[l * W16 + k] = Y[k][l]
-    GT* Rs = Ys + W16 * W16;                 // W16 x 64  [k][col]
-    GT* Zs = Rs + W16 * 64;                  // W16 x 65  [k][col]
-    GT* Cs = Zs + W16 * 65;                  // 64 x W16+1 [row][k]
-    // site of flip `lane` (lanes >= m: any valid row, masked below)
-    const int sk = S[1 + (lane < m ? lane : 0)];
+    GT* Ys = reinterpret_cast<GT*>(smem);   // W x W column-major: Ys[l * W + k] = Y[k][l]
+    GT* Rs = Ys + W * W;                     // W x 64  [k][col]
+    GT* Zs = Rs + W * 64;                    // W x 65  [k][col]
+    GT* Cs = Zs + W * 65;                    // 64 x W+1 [row][k]
     {
-      // R[k][c0 + cc] = Gs[s_k][c0 + cc]: lane = k, cc = wave + 4 q
+      // R[k][c0 + cc] = Gs[wp + k][c0 + cc]: lane = k, cc = wave + 4 q
+      const int k = wp + (lane < W ? lane : W - 1);
+      const size_t ko = (size_t)(k < n ? k : n - 1);
       GT v[16];
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int gc = c0 + wave + 4 * q;
-        v[q] = Gb[(size_t)sk + (size_t)(gc < n ? gc : n - 1) * n];
+        v[q] = Gb[ko + (size_t)(gc < n ? gc : n - 1) * n];
       }
 #pragma unroll
       for (int q = 0; q < 16; q++) {
         const int gc = c0 + wave + 4 * q;
-        if (lane < W16) Rs[lane * 64 + wave + 4 * q] = (lane < m && gc < n) ? v[q] : GT(0);
+        if (lane < W) Rs[lane * 64 + wave + 4 * q] = gc < n ? v[q] : GT(0);
       }
     }
     {
-      // C[r0 + rr][k] = delta - Gs[r0 + rr][s_k]: lane = rr, k = wave + 4 q (uniform per wave)
-      constexpr int QC = W16 / 4;
+      // C[r0 + rr][k] = delta - Gs[r0 + rr][wp + k]: lane = rr, k = wave + 4 q
+      constexpr int QC = W / 4;
       const int gr = r0 + lane;
       const int grc = gr < n ? gr : n - 1;
       GT v[QC];
-      int skq[QC];
 #pragma unroll
       for (int q = 0; q < QC; q++) {
-        const int k = wave + 4 * q;
-        skq[q] = __builtin_amdgcn_readlane(sk, k < 64 ? k : 0);
-        v[q] = Gb[(size_t)grc + (size_t)skq[q] * n];
+        const int k = wp + wave + 4 * q;
+        v[q] = Gb[(size_t)grc + (size_t)(k < n ? k : n - 1) * n];
       }
 #pragma unroll
       for (int q = 0; q < QC; q++) {
         const int k = wave + 4 * q;
-        Cs[lane * (W16 + 1) + k] = (k < m && gr < n) ? (gr == skq[q] ? GT(1) : GT(0)) - v[q] : GT(0);
+        Cs[lane * (W + 1) + k] = gr < n ? (gr == wp + k ? GT(1) : GT(0)) - v[q] : GT(0);
       }
     }
     {
-      // Y (row-major W x W in global): Ys[l * W16 + k] = Y[k][l]; lane = l, k = wave + 4 q
-      constexpr int QY = W16 / 4;
+      // Y (row-major W x W in global): Ys[l * W + k] = Y[k][l]; lane = l, k = wave + 4 q
+      constexpr int QY = W / 4;
       const int lc = lane < W ? lane : W - 1;
       GT v[QY];
 #pragma unroll
-      for (int q = 0; q < QY; q++) {
-        const int k = wave + 4 * q;
-        v[q] = Y[(size_t)b * W * W + (size_t)(k < W ? k : W - 1) * W + lc];
-      }
+      for (int q = 0; q < QY; q++) v[q] = Y[(size_t)b * W * W + (size_t)(wave + 4 * q) * W + lc];
 #pragma unroll
-      for (int q = 0; q < QY; q++) {
-        const int k = wave + 4 * q;
-        if (lane < W16) Ys[lane * W16 + k] = (k < W && lane < W) ? v[q] : GT(0);
-      }
+      for (int q = 0; q < QY; q++)
+        if (lane < W) Ys[lane * W + wave + 4 * q] = v[q];
     }
     __syncthreads();
-    // Z[:, 16 wave : 16 wave + 16] = Y R[:, same columns], rows and k up to m16
+    // Z[:, 16 wave : 16 wave + 16] = Y R[:, same columns]
 #pragma unroll
-    for (int i0 = 0; i0 < W16; i0 += 16) {
-      if (i0 < m16) {
-        acc_t z = {0, 0, 0, 0};
+    for (int i0 = 0; i0 < W; i0 += 16) {
+      acc_t z = {0, 0, 0, 0};
 #pragma unroll
-        for (int k0 = 0; k0 < W16; k0 += 4) {
-          if (k0 < m16) {
-            const GT av = Ys[(k0 + lk) * W16 + i0 + li];       // Y[i0 + li][k0 + lk]
-            const GT bv = Rs[(k0 + lk) * 64 + 16 * wave + li];  // R[k0 + lk][16 wave + li]
-            z = mfma16(av, bv, z);
-          }
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) Zs[(i0 + mfma_row<GT>(lane, r)) * 65 + 16 * wave + li] = z[r];
+      for (int k0 = 0; k0 < W; k0 += 4) {
+        const GT av = Ys[(k0 + lk) * W + i0 + li];        // Y[i0 + li][k0 + lk]
+        const GT bv = Rs[(k0 + lk) * 64 + 16 * wave + li];  // R[k0 + lk][16 wave + li]
+        z = mfma16(av, bv, z);
       }
+#pragma unroll
+      for (int r = 0; r < 4; r++) Zs[(i0 + mfma_row<GT>(lane, r)) * 65 + 16 * wave + li] = z[r];
     }
     __syncthreads();
 #pragma unroll
-    for (int k0 = 0; k0 < W16; k0 += 4) {
-      if (k0 < m16) {
-        const int k = k0 + lk;
-        const GT bv = Cs[(rr0 + li) * (W16 + 1) + k];       // C^T[k][row]
+    for (int k0 = 0; k0 < W; k0 += 4) {
+      const int k = k0 + lk;
+      const GT bv = Cs[(rr0 + li) * (W + 1) + k];          // C^T[k][row]
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const GT av = -Zs[k * 65 + 16 * j + li];          // Z^T[col][k]
-          acc[j] = mfma16(av, bv, acc[j]);
-        }
+      for (int j = 0; j < 4; j++) {
+        const GT av = -Zs[k * 65 + 16 * j + li];           // Z^T[col][k]
+        acc[j] = mfma16(av, bv, acc[j]);
       }
     }
   }
@@ -473,35 +547,29 @@ __global__ __launch_bounds__(256) void window_apply_kernel(GT* __restrict__ Gd, 
     }
 }
 
-template <typename GT>
-static size_t apply_lds_bytes(int W16) {
-  return sizeof(GT) * ((size_t)W16 * W16 + W16 * 64 + W16 * 65 + 64 * (W16 + 1));
+// One launch per window: workgroup 0 runs the window (decisions on the block formed from
+// the previous window's factors), workgroups 1.. apply the previous window's flips to all
+// of G, tile by tile, into the other copy.
+template <typename GT, int FLV, int W>
+__global__ __launch_bounds__(256) void window_kernel(WindowArgs<GT> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (blockIdx.x == 0) {
+    window_role<GT, FLV, W>(a, smem);
+    return;
+  }
+  apply_tile<GT, W>(a.Gd, a.G, a.Sp, a.Yp, a.n, a.wp, (int)blockIdx.x - 1, smem);
 }
 
-template <typename GT, int W16>
-static void launch_apply_w(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
-                           hipStream_t s) {
-  dim3 grid((n + 63) / 64, (n + 63) / 64, flv);
-  hipLaunchKernelGGL((window_apply_kernel<GT, W16>), grid, dim3(256), apply_lds_bytes<GT>(W16), s, Gd, Gs, S,
-                     Y, n, W);
+// The last window's application alone.
+template <typename GT, int W>
+__global__ __launch_bounds__(256) void window_apply_kernel(WindowArgs<GT> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  apply_tile<GT, W>(a.Gd, a.G, a.Sp, a.Yp, a.n, a.wp, (int)blockIdx.x, smem);
 }
 
-template <typename GT>
-static hipError_t launch_apply_t(GT* Gd, const GT* Gs, const int* S, const GT* Y, int n, int W, int flv,
-                                 hipStream_t s) {
-  if (W != 16 && W != 32 && W != 64) return hipErrorInvalidValue;
-  if (W == 16) launch_apply_w<GT, 16>(Gd, Gs, S, Y, n, W, flv, s);
-  else if (W == 32) launch_apply_w<GT, 32>(Gd, Gs, S, Y, n, W, flv, s);
-  else launch_apply_w<GT, 64>(Gd, Gs, S, Y, n, W, flv, s);
-  return hipGetLastError();
-}
-hipError_t launch_window_apply_f64(double* Gd, const double* Gs, const int* S, const double* Y, int n, int W,
-                                   int flv, hipStream_t s) {
-  return launch_apply_t(Gd, Gs, S, Y, n, W, flv, s);
-}
-hipError_t launch_window_apply_f32(float* Gd, const float* Gs, const int* S, const float* Y, int n, int W,
-                                   int flv, hipStream_t s) {
-  return launch_apply_t(Gd, Gs, S, Y, n, W, flv, s);
+template <typename GT, int W>
+constexpr size_t apply_lds() {
+  return sizeof(GT) * ((size_t)W * W + W * 64 + W * 65 + 64 * (W + 1));
 }
 
 size_t window_lds_bytes(int W, int flv) {
@@ -518,12 +586,23 @@ int window_size(int n, int flv) {
 
 template <typename GT, int FLV, int W>
 static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
-  hipLaunchKernelGGL((window_kernel<GT, FLV, W>), dim3(1), dim3(256), (window_lds<FLV, W>()), s, a);
+  const int nt = (a.n + 63) / 64;
+  const int tiles = a.Yp ? FLV * nt * nt : 0;
+  if (a.do_window) {
+    const size_t lds = tiles && apply_lds<GT, W>() > window_lds<FLV, W>() ? apply_lds<GT, W>()
+                                                                           : window_lds<FLV, W>();
+    hipLaunchKernelGGL((window_kernel<GT, FLV, W>), dim3(1 + tiles), dim3(256), lds, s, a);
+  } else if (tiles) {
+    hipLaunchKernelGGL((window_apply_kernel<GT, W>), dim3(tiles), dim3(256), (apply_lds<GT, W>()), s, a);
+  }
 }
 
 template <typename GT>
 static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
   if (a.flv < 1 || a.flv > 2 || (a.W != 16 && a.W != 32 && a.W != 64)) return hipErrorInvalidValue;
+  if (a.Yp && (a.Sp == nullptr || a.Gd == nullptr || a.Gd == a.G || a.wp < 0 || a.wp >= a.n))
+    return hipErrorInvalidValue;
+  if (a.do_window && (a.w0 < 0 || a.w0 >= a.n || (a.Yp && a.wp + a.W != a.w0))) return hipErrorInvalidValue;
   if (a.flv == 1) {
     if (a.W == 16) launch_window_w<GT, 1, 16>(a, s);
     else if (a.W == 32) launch_window_w<GT, 1, 32>(a, s);
@@ -540,21 +619,21 @@ hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s) { return
 
 hipError_t local_kernels_init() {
   const void* fns[] = {reinterpret_cast<const void*>(window_kernel<double, 1, 16>),
-                       reinterpret_cast<const void*>(window_kernel<double, 2, 16>),
-                       reinterpret_cast<const void*>(window_kernel<float, 1, 16>),
-                       reinterpret_cast<const void*>(window_kernel<float, 2, 16>),
-                       reinterpret_cast<const void*>(window_apply_kernel<double, 16>),
-                       reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
                        reinterpret_cast<const void*>(window_kernel<double, 1, 32>),
                        reinterpret_cast<const void*>(window_kernel<double, 1, 64>),
+                       reinterpret_cast<const void*>(window_kernel<double, 2, 16>),
                        reinterpret_cast<const void*>(window_kernel<double, 2, 32>),
                        reinterpret_cast<const void*>(window_kernel<double, 2, 64>),
+                       reinterpret_cast<const void*>(window_kernel<float, 1, 16>),
                        reinterpret_cast<const void*>(window_kernel<float, 1, 32>),
                        reinterpret_cast<const void*>(window_kernel<float, 1, 64>),
+                       reinterpret_cast<const void*>(window_kernel<float, 2, 16>),
                        reinterpret_cast<const void*>(window_kernel<float, 2, 32>),
                        reinterpret_cast<const void*>(window_kernel<float, 2, 64>),
+                       reinterpret_cast<const void*>(window_apply_kernel<double, 16>),
                        reinterpret_cast<const void*>(window_apply_kernel<double, 32>),
                        reinterpret_cast<const void*>(window_apply_kernel<double, 64>),
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 32>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 64>)};
   for (const void* f : fns) {
