@@ -179,6 +179,5 @@ struct WindowArgs {
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
 int window_size(int n, int flv);
-size_t window_lds_bytes(int W, int flv);
 
 }  // namespace dqmc

@@ -103,9 +103,15 @@ __device__ __forceinline__ double readlane_f64(double x, int l) {
 template <int W>
 __device__ __forceinline__ int vpos(int c) { return (c & 3) * (W / 4) + (c >> 2); }
 
-template <int FLV, int W>
+// LDS of the window workgroup: the decision loop's Uw, Vw, gamma, sites; before the loop,
+// the block Bn (over Uw) and, for W <= 32, the staged operands and FP32 Z scratch after it.
+template <typename GT, int FLV, int W>
 constexpr size_t window_lds() {
-  return (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4;
+  const size_t loop = (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4 + 16;
+  const size_t block = W > 32 ? 0
+                              : (size_t)FLV * W * (W + 1) * 8 + (size_t)4 * FLV * W * (W + 1) * sizeof(GT) +
+                                    (sizeof(GT) == 4 ? (size_t)4 * W * 17 * 4 : 0);
+  return loop > block ? loop : block;
 }
 
 // The window's block of G after the previous window's flips, from the source copy:
@@ -114,9 +120,14 @@ constexpr size_t window_lds() {
 // cores with the operands, order and roles of the application's tiles, so that B equals
 // the block the application writes.  Wave unit (b, ci) forms the 16-column tile ci of
 // flavour block b; the result goes to LDS as Bn[b][c][r] (leading dimension W + 1).
+//
+// W <= 32: the four W x W operands of every block (G[w, w], G[w, wp], G[wp, w] and the
+// previous factor) are first staged in LDS with coalesced loads (lane = row of a column),
+// all in flight together.  W = 64 (no room to stage): operands are loaded per unit
+// straight into the matrix-core registers.
 template <typename GT, int FLV, int W>
-__device__ __forceinline__ void window_block(const WindowArgs<GT>& a, double* Bn, GT* Zs, int Wn, int lane,
-                                             int wave) {
+__device__ __forceinline__ void window_block_direct(const WindowArgs<GT>& a, double* Bn, GT* Zs, int Wn,
+                                                    int lane, int wave) {
   typedef typename V4<GT>::type acc_t;
   constexpr int CT = W / 16, KS = W / 4, LD = W + 1;
   const int n = a.n, w0 = a.w0, wp = a.wp;
@@ -199,6 +210,109 @@ __device__ __forceinline__ void window_block(const WindowArgs<GT>& a, double* Bn
 }
 
 template <typename GT, int FLV, int W>
+__device__ __forceinline__ void window_block_staged(const WindowArgs<GT>& a, double* Bn, GT* St, GT* Zw, int Wn,
+                                                    int tid, int lane, int wave) {
+  typedef typename V4<GT>::type acc_t;
+  constexpr int CT = W / 16, KS = W / 4, LD = W + 1, WW = W * W, QM = WW / 256;
+  static_assert(WW % 256 == 0, "staging assumes whole passes of 256 threads per matrix");
+  const int n = a.n, w0 = a.w0, wp = a.wp;
+  const size_t nn = (size_t)n * n;
+  const int li = lane & 15, lk = lane >> 4;
+  const bool corr = a.Yp != nullptr;
+  const int mp = corr ? a.Sp[0] : 0;
+  // St[(4 b + mat) W LD + c LD + r]: mat 0 = G[w, w], 1 = G[w, wp], 2 = G[wp, w]
+  // (column-major, [c][r]); mat 3 = previous factor (row-major, [p][q])
+  GT v[FLV][4][QM];
+#pragma unroll
+  for (int b = 0; b < FLV; b++) {
+    const GT* Gb = a.G + (size_t)b * nn;
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int idx = tid + 256 * q;
+      const int r = idx % W, c = idx / W;
+      const int rw = w0 + (r < Wn ? r : Wn - 1), cw = w0 + (c < Wn ? c : Wn - 1);
+      const int rp = wp + r < n ? wp + r : n - 1, cp = wp + c < n ? wp + c : n - 1;
+      v[b][0][q] = Gb[(size_t)rw + (size_t)cw * n];
+      if (corr) {
+        v[b][1][q] = Gb[(size_t)rw + (size_t)cp * n];
+        v[b][2][q] = Gb[(size_t)rp + (size_t)cw * n];
+        v[b][3][q] = a.Yp[(size_t)b * WW + idx];
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < FLV; b++)
+#pragma unroll
+    for (int q = 0; q < QM; q++) {
+      const int idx = tid + 256 * q;
+      const int r = idx % W, c = idx / W;
+      GT* Sb = St + (size_t)(4 * b) * W * LD + c * LD + r;
+      Sb[0] = v[b][0][q];
+      if (corr) {
+        Sb[W * LD] = v[b][1][q];
+        Sb[2 * W * LD] = v[b][2][q];
+        Sb[3 * W * LD] = v[b][3][q];
+      }
+    }
+  __syncthreads();
+  for (int unit = wave; unit < FLV * CT; unit += 4) {
+    const int b = unit / CT, ci = unit - b * CT;
+    const GT* Bs = St + (size_t)(4 * b) * W * LD;
+    const GT* Ps = Bs + W * LD;
+    const GT* Qs = Bs + 2 * W * LD;
+    const GT* Ys = Bs + 3 * W * LD;
+    const int cc = ci * 16 + li;
+    acc_t o[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) o[t][r] = Bs[cc * LD + t * 16 + mfma_row<GT>(lane, r)];
+    if (corr && mp > 0) {
+      acc_t z[CT];
+#pragma unroll
+      for (int t = 0; t < CT; t++) {
+        const acc_t zero = {0, 0, 0, 0};
+        z[t] = zero;
+#pragma unroll
+        for (int s = 0; s < KS; s++)                                   // Z = Y G[wp, w]
+          z[t] = mfma16(Ys[(t * 16 + li) * LD + 4 * s + lk], Qs[cc * LD + 4 * s + lk], z[t]);
+      }
+      if constexpr (sizeof(GT) == 8) {
+        // element r of Z tile t is Z[16 t + lk + 4 r][cc]: the B operand of k-step 4 t + r
+#pragma unroll
+        for (int t = 0; t < CT; t++)
+#pragma unroll
+          for (int s = 0; s < KS; s++) o[t] = mfma16(Ps[(4 * s + lk) * LD + t * 16 + li], z[s >> 2][s & 3], o[t]);
+      } else {
+        // FP32 accumulator rows are 4 (lane / 16) + r: regroup Z through this wave's scratch
+        GT* Zb = Zw + wave * W * 17;
+#pragma unroll
+        for (int t = 0; t < CT; t++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) Zb[(t * 16 + mfma_row<GT>(lane, r)) * 17 + li] = z[t][r];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+          const GT bz = Zb[(4 * s + lk) * 17 + li];
+#pragma unroll
+          for (int t = 0; t < CT; t++) o[t] = mfma16(Ps[(4 * s + lk) * LD + t * 16 + li], bz, o[t]);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+    double* Bb = Bn + (size_t)b * W * LD;
+#pragma unroll
+    for (int t = 0; t < CT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) Bb[cc * LD + t * 16 + mfma_row<GT>(lane, r)] = (double)o[t][r];
+  }
+}
+
+template <typename GT, int FLV, int W>
 __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned char* smem) {
   constexpr int NW = 4, CPW = W / NW, LD = W + 1;
   typedef double colvec __attribute__((ext_vector_type(CPW)));
@@ -208,6 +322,7 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   double* Vw = Uw + FLV * W * LD;
   double* gam = Vw + FLV * W * LD;                        // [b][l]
   int* sites = reinterpret_cast<int*>(gam + FLV * W);     // [l], window index of flip l
+  unsigned long long* shm = reinterpret_cast<unsigned long long*>(sites + W);   // m, accepted mask
   const int n = a.n, w0 = a.w0;
   const int Wn = n - w0 < W ? n - w0 : W;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -226,119 +341,244 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   // window block after the previous window's flips (LDS, aliasing Uw), then into registers:
   // lane = row, wave w keeps columns w, w + 4, ...; every wave keeps the diagonal (lane p: G_pp)
   double* Bn = Uw;
-  window_block<GT, FLV, W>(a, Bn, reinterpret_cast<GT*>(Vw), Wn, lane, wave);
-  __syncthreads();
-  colvec g0, g1;
-#pragma unroll
-  for (int i = 0; i < CPW; i++) {
-    const int c = wave + NW * i;
-    const int cc = c < Wn ? c : Wn - 1;
-    const double x0 = Bn[cc * LD + lr];
-    const double x1 = FLV > 1 ? Bn[(W + cc) * LD + lr] : 0.0;
-    const bool ok = rowok && c < Wn;
-    g0[i] = ok ? x0 : 0.0;
-    g1[i] = ok ? x1 : 0.0;
+  if constexpr (W <= 32) {
+    GT* St = reinterpret_cast<GT*>(Uw + FLV * W * LD);
+    window_block_staged<GT, FLV, W>(a, Bn, St, St + 4 * FLV * W * LD, Wn, tid, lane, wave);
+  } else {
+    window_block_direct<GT, FLV, W>(a, Bn, reinterpret_cast<GT*>(Vw), Wn, lane, wave);
   }
-  double d0 = rowok ? Bn[lr * LD + lr] : 0.0;
-  double d1 = (FLV > 1 && rowok) ? Bn[(W + lr) * LD + lr] : 0.0;
-  __syncthreads();                  // Bn is overwritten by the first flip's u, v
-
+  __syncthreads();
   const double df0 = a.d_first[0], df1 = a.d_first[1];
   const double ds0 = a.d_second[0], ds1 = a.d_second[1];
   const double bw0 = a.bweight[0], bw1 = a.bweight[1];
   int used = 0;                    // uniforms consumed in this window
   int neg = 0, ties = 0;
-  int m = 0;
+  int m = 0;                       // flips accepted (every wave, after the loop)
   // per lane: decision and p of site `lane`; site and gammas of flip `lane`
   int acc_r = 0, site_r = 0;
   double p_r = 0.0, gam0_r = 0.0, gam1_r = 0.0;
 
-  for (int t = 0; t < Wn; t++) {
-    const int x = __builtin_amdgcn_readlane(cfl, t);
-    const int ix = x > 0 ? 1 : 0;
-    const double g0d = readlane_f64(d0, t);
-    const double g1d = FLV > 1 ? readlane_f64(d1, t) : 0.0;
-    const Proposal q = propose(FLV, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
-    const double p = q.p;
-    bool accept;
-    neg += p < 0.0;
-    if (p > 1.0) {
-      accept = true;
-      ties += (p - 1.0 < 1e-12);
-    } else {
-      const double u = readlane_f64(ul, used);
-      used++;
-      accept = u < p;
-      ties += (fabs(u - p) < 1e-12);
-    }
-    p_r = lane == t ? p : p_r;
-    acc_r = lane == t ? (int)accept : acc_r;
-    if (!accept) continue;
-
-    double* U0 = Uw + m * LD;
-    double* U1 = Uw + (W + m) * LD;
-    double* V0 = Vw + m * LD;
-    double* V1 = Vw + (W + m) * LD;
-    if (wave == (t & (NW - 1))) {           // owner of column t: u = e_t - G[:, t]
-      const int it = t >> 2;
-      const double c0 = g0[it];
-      const double c1 = FLV > 1 ? g1[it] : 0.0;
-      if (lane < W) {
-        const double e = lane == t ? 1.0 : 0.0;
-        U0[lane] = e - c0;
-        if (FLV > 1) U1[lane] = e - c1;
-      }
-    }
-    if (lane == t) {                        // row t on this wave's columns
+  if constexpr (W <= 32) {
+    // Single-wave decisions: wave 0 lane 32 f + r holds row r of flavour block f, all W
+    // columns in registers; u of an accepted flip at site t is each lane's own G[r][t],
+    // and row t reaches the other lanes through LDS inside the wave (no workgroup barrier).
+    // Waves 1-3 wait at the barrier after the loop.
+    typedef double rowvec __attribute__((ext_vector_type(16)));
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const int f = lane >> 5, r = lane & 31;
+    const bool fok = f < FLV;
+    const int fb = fok ? f : 0;          // lanes of an absent second block read block 0, unused
+    if (wave == 0) {
+      const bool rok = fok && r < Wn;
+      const int rc = r < Wn ? r : Wn - 1;
+      const double* Bf = Bn + (size_t)fb * W * LD;
+      rowvec lo, hi;
 #pragma unroll
-      for (int i = 0; i < CPW; i++) {
-        V0[wave * CPW + i] = g0[i];
-        if (FLV > 1) V1[wave * CPW + i] = g1[i];
+      for (int c = 0; c < 16; c++) {
+        const int cl = c < Wn ? c : Wn - 1;
+        const double xl = Bf[cl * LD + rc];
+        lo[c] = (rok && c < Wn) ? xl : 0.0;
+        if constexpr (W > 16) {
+          const int ch = c + 16 < Wn ? c + 16 : Wn - 1;
+          const double xh = Bf[ch * LD + rc];
+          hi[c] = (rok && c + 16 < Wn) ? xh : 0.0;
+        } else {
+          hi[c] = 0.0;
+        }
+      }
+      double dg = rok ? Bf[rc * LD + rc] : 0.0;
+      // order the reads of Bn before the first flip's stores over it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_s_waitcnt(0xC07F);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (int t = 0; t < Wn; t++) {
+        const int x = __builtin_amdgcn_readlane(cfl, t);
+        const int ix = x > 0 ? 1 : 0;
+        const double g0d = readlane_f64(dg, t);
+        const double g1d = FLV > 1 ? readlane_f64(dg, 32 + t) : 0.0;
+        const Proposal q = propose(FLV, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
+        const double p = q.p;
+        bool accept;
+        neg += p < 0.0;
+        if (p > 1.0) {
+          accept = true;
+          ties += (p - 1.0 < 1e-12);
+        } else {
+          const double u = readlane_f64(ul, used);
+          used++;
+          accept = u < p;
+          ties += (fabs(u - p) < 1e-12);
+        }
+        p_r = lane == t ? p : p_r;
+        acc_r = lane == t ? (int)accept : acc_r;
+        if (!accept) continue;
+
+        const double gt = (W <= 16 || t < 16) ? lo[t & 15] : hi[t & 15];   // G_f[r][t]
+        double* Vr = Vw + (size_t)(fb * W + m) * W;                        // raw row t
+        if (fok && r == t) {
+#pragma unroll
+          for (int c = 0; c < 16; c += 2) {
+            *reinterpret_cast<d2*>(Vr + c) = d2{lo[c], lo[c + 1]};
+            if constexpr (W > 16) *reinterpret_cast<d2*>(Vr + 16 + c) = d2{hi[c], hi[c + 1]};
+          }
+        }
+        const double uraw = (r == t ? 1.0 : 0.0) - gt;                     // u_t(r), unscaled
+        if (fok && r < W) Uw[(size_t)(f * W + m) * LD + r] = uraw;
+        site_r = lane == m ? t : site_r;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // this lane's block: gamma = Delta_f / R_f (one division, overlapping the row reads)
+        const double gamL = gamma_of(f ? q.D1 : q.D0, f ? q.R1 : q.R0);
+        if (fok && r == m) gam[f * W + m] = gamL;
+        const double ug = fok ? gamL * uraw : 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; c += 2) {
+          const d2 vl = *reinterpret_cast<const d2*>(Vr + c);
+          lo[c] = fma(-ug, vl[0], lo[c]);
+          lo[c + 1] = fma(-ug, vl[1], lo[c + 1]);
+          if constexpr (W > 16) {
+            const d2 vh = *reinterpret_cast<const d2*>(Vr + 16 + c);
+            hi[c] = fma(-ug, vh[0], hi[c]);
+            hi[c + 1] = fma(-ug, vh[1], hi[c + 1]);
+          }
+        }
+        dg = fma(-ug, Vr[r < W ? r : W - 1], dg);
+        m++;
+      }
+      const unsigned long long am = __ballot(acc_r != 0);    // whole wave active here
+      if (lane == 0) {
+        shm[0] = (unsigned long long)m;
+        shm[1] = am;
+      }
+      if (lane < m) sites[lane] = site_r;
+      else if (lane < W) {
+        sites[lane] = 0;
+        gam[lane] = 0.0;                  // rows beyond m come out exactly zero
+        if (FLV > 1) gam[W + lane] = 0.0;
+      }
+      if (lane < Wn) {
+        if (a.dec_log && log0 + lane < a.log_cap) {
+          a.dec_log[log0 + lane] = (unsigned char)acc_r;
+          if (a.p_log) a.p_log[log0 + lane] = p_r;
+        }
+        if (acc_r) a.conf[w0 + lane] = (int8_t)(-cfl);
       }
     }
-    const double gam0 = gamma_of(q.D0, q.R0);
-    const double gam1 = FLV > 1 ? gamma_of(q.D1, q.R1) : 0.0;
-    site_r = lane == m ? t : site_r;
-    gam0_r = lane == m ? gam0 : gam0_r;
-    gam1_r = lane == m ? gam1 : gam1_r;
     __syncthreads();
-    // G -= (gamma u) row_t on this wave's columns and on the diagonal copy
-    const int lw = lane < W ? lane : W - 1;
-    const double u0 = lane < W ? gam0 * U0[lw] : 0.0;
-    const double u1 = (FLV > 1 && lane < W) ? gam1 * U1[lw] : 0.0;
+    m = (int)shm[0];
+  } else {
+    colvec g0, g1;
 #pragma unroll
     for (int i = 0; i < CPW; i++) {
-      g0[i] = fma(-u0, V0[wave * CPW + i], g0[i]);
-      if (FLV > 1) g1[i] = fma(-u1, V1[wave * CPW + i], g1[i]);
+      const int c = wave + NW * i;
+      const int cc = c < Wn ? c : Wn - 1;
+      const double x0 = Bn[cc * LD + lr];
+      const double x1 = FLV > 1 ? Bn[(W + cc) * LD + lr] : 0.0;
+      const bool ok = rowok && c < Wn;
+      g0[i] = ok ? x0 : 0.0;
+      g1[i] = ok ? x1 : 0.0;
     }
-    d0 = fma(-u0, V0[vpos<W>(lw)], d0);
-    if (FLV > 1) d1 = fma(-u1, V1[vpos<W>(lw)], d1);
-    m++;
-  }
+    double d0 = rowok ? Bn[lr * LD + lr] : 0.0;
+    double d1 = (FLV > 1 && rowok) ? Bn[(W + lr) * LD + lr] : 0.0;
+    __syncthreads();                  // Bn is overwritten by the first flip's u, v
 
-  if (wave == 0) {
-    if (lane < m) {
-      sites[lane] = site_r;
-      gam[lane] = gam0_r;
-      if (FLV > 1) gam[W + lane] = gam1_r;
-    } else if (lane < W) {
-      sites[lane] = 0;
-      gam[lane] = 0.0;                  // rows beyond m come out exactly zero
-      if (FLV > 1) gam[W + lane] = 0.0;
-    }
-    if (lane < Wn) {
-      if (a.dec_log && log0 + lane < a.log_cap) {
-        a.dec_log[log0 + lane] = (unsigned char)acc_r;
-        if (a.p_log) a.p_log[log0 + lane] = p_r;
+    for (int t = 0; t < Wn; t++) {
+      const int x = __builtin_amdgcn_readlane(cfl, t);
+      const int ix = x > 0 ? 1 : 0;
+      const double g0d = readlane_f64(d0, t);
+      const double g1d = FLV > 1 ? readlane_f64(d1, t) : 0.0;
+      const Proposal q = propose(FLV, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
+      const double p = q.p;
+      bool accept;
+      neg += p < 0.0;
+      if (p > 1.0) {
+        accept = true;
+        ties += (p - 1.0 < 1e-12);
+      } else {
+        const double u = readlane_f64(ul, used);
+        used++;
+        accept = u < p;
+        ties += (fabs(u - p) < 1e-12);
       }
-      if (acc_r) a.conf[w0 + lane] = (int8_t)(-cfl);
-    }
-  }
-  __syncthreads();
+      p_r = lane == t ? p : p_r;
+      acc_r = lane == t ? (int)accept : acc_r;
+      if (!accept) continue;
 
-  // Gather the m x m coefficient blocks, row-major, over the (now free) Uw/Vw storage:
-  //   Lc[b][j][l] = u_l(s_j) (l < j),  Nc[b][j][l] = G_j[s_j][s_l] (j < l), zero elsewhere.
-  double* Lc = Uw;                 // Lc[b] at 2 b W^2, Nc[b] at (2 b + 1) W^2
+      double* U0 = Uw + m * LD;
+      double* U1 = Uw + (W + m) * LD;
+      double* V0 = Vw + m * LD;
+      double* V1 = Vw + (W + m) * LD;
+      if (wave == (t & (NW - 1))) {           // owner of column t: u = e_t - G[:, t]
+        const int it = t >> 2;
+        const double c0 = g0[it];
+        const double c1 = FLV > 1 ? g1[it] : 0.0;
+        if (lane < W) {
+          const double e = lane == t ? 1.0 : 0.0;
+          U0[lane] = e - c0;
+          if (FLV > 1) U1[lane] = e - c1;
+        }
+      }
+      if (lane == t) {                        // row t on this wave's columns
+#pragma unroll
+        for (int i = 0; i < CPW; i++) {
+          V0[wave * CPW + i] = g0[i];
+          if (FLV > 1) V1[wave * CPW + i] = g1[i];
+        }
+      }
+      __syncthreads();
+      // G -= (gamma u) row_t on this wave's columns and on the diagonal copy; the divisions
+      // overlap the reads of u
+      const int lw = lane < W ? lane : W - 1;
+      const double U0v = U0[lw];
+      const double U1v = FLV > 1 ? U1[lw] : 0.0;
+      const double gam0 = gamma_of(q.D0, q.R0);
+      const double gam1 = FLV > 1 ? gamma_of(q.D1, q.R1) : 0.0;
+      site_r = lane == m ? t : site_r;
+      gam0_r = lane == m ? gam0 : gam0_r;
+      gam1_r = lane == m ? gam1 : gam1_r;
+      const double u0 = lane < W ? gam0 * U0v : 0.0;
+      const double u1 = (FLV > 1 && lane < W) ? gam1 * U1v : 0.0;
+#pragma unroll
+      for (int i = 0; i < CPW; i++) {
+        g0[i] = fma(-u0, V0[wave * CPW + i], g0[i]);
+        if (FLV > 1) g1[i] = fma(-u1, V1[wave * CPW + i], g1[i]);
+      }
+      d0 = fma(-u0, V0[vpos<W>(lw)], d0);
+      if (FLV > 1) d1 = fma(-u1, V1[vpos<W>(lw)], d1);
+      m++;
+    }
+
+    if (wave == 0) {
+      if (lane < m) {
+        sites[lane] = site_r;
+        gam[lane] = gam0_r;
+        if (FLV > 1) gam[W + lane] = gam1_r;
+      } else if (lane < W) {
+        sites[lane] = 0;
+        gam[lane] = 0.0;                  // rows beyond m come out exactly zero
+        if (FLV > 1) gam[W + lane] = 0.0;
+      }
+      if (lane < Wn) {
+        if (a.dec_log && log0 + lane < a.log_cap) {
+          a.dec_log[log0 + lane] = (unsigned char)acc_r;
+          if (a.p_log) a.p_log[log0 + lane] = p_r;
+        }
+        if (acc_r) a.conf[w0 + lane] = (int8_t)(-cfl);
+      }
+    }
+    if (wave == 0) {
+      const unsigned long long am = __ballot(acc_r != 0);
+      if (lane == 0) shm[1] = am;
+    }
+    __syncthreads();
+  }
+  const unsigned long long accm = shm[1];    // accepted window positions
+
+  // Gather the m x m coefficient blocks, transposed, over the (now free) Uw/Vw storage:
+  //   Lt[b][l][j] = u_l(s_j) (l < j),  Nt[b][l][j] = gamma_j G_j[s_j][s_l] (j < l),
+  // zero elsewhere: column l of L and of Gamma N is a contiguous row.
+  double* Lc = Uw;                 // Lt[b] at 2 b W^2, Nt[b] at (2 b + 1) W^2
   if (m > 0) {
     constexpr int E = 2 * FLV * W * W / 256;
     double gv[E];
@@ -346,14 +586,15 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
     for (int q = 0; q < E; q++) {
       const int e = tid + 256 * q;
       const int mat = e / (W * W), rem = e - mat * W * W;
-      const int j = rem / W, l = rem - j * W;
+      const int l = rem / W, j = rem - l * W;
       const int b = mat >> 1;
       // unconditional in-range loads (sites[] is 0 beyond m), then the mask
       const double vl = Uw[(b * W + l) * LD + sites[j]];
-      const double vn = Vw[(b * W + j) * LD + vpos<W>(sites[l])];
+      const double vn = W <= 32 ? Vw[(b * W + j) * W + sites[l]] : Vw[(b * W + j) * LD + vpos<W>(sites[l])];
+      const double gj = gam[b * W + j];
       const bool isl = (mat & 1) == 0;
       const bool keep = isl ? (l < j && j < m) : (j < l && l < m);
-      gv[q] = keep ? (isl ? vl : vn) : 0.0;
+      gv[q] = keep ? (isl ? vl : gj * vn) : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -362,52 +603,75 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   }
 
   // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma, wave b for flavour block b, lane = column c,
-  // rows in registers; four partial sums per dot product.
+  // column c of Y in registers.  Column-oriented substitutions: once x_j (y_l) is final it
+  // is subtracted from (added to) the remaining rows, so each step's own chain is one
+  // multiply and one fma; the next column of coefficients is loaded during the step.
   if (wave < FLV && m > 0) {
     const int b = wave;
-    const double* Lb = Lc + 2 * b * W * W;
-    const double* Nb = Lc + (2 * b + 1) * W * W;
+    const double* Lt = Lc + 2 * b * W * W;
+    const double* Nt = Lc + (2 * b + 1) * W * W;
     const double* gb = gam + b * W;
-    double xr[W];
-    // forward: x_j = gamma_j (delta_jc - sum_{l<j} u_l(s_j) x_l)
+    double xr[W], cc[W], cn[W] = {};
+#pragma unroll
+    for (int k = 0; k < W; k++) xr[k] = lane == k ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 1; k < W; k++) cc[k] = Lt[k];
+    double gc = gb[0], gn = 0.0;
+    // forward: x_j = gamma_j s_j, then s_k -= L[k][j] x_j for k > j
 #pragma unroll
     for (int j = 0; j < W; j++) {
-      if (j < m) {
-        double s4[4] = {lane == j ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+      if (j + 1 < W) {                   // unconditional: columns beyond m are zero
 #pragma unroll
-        for (int l = 0; l < j; l++) s4[l & 3] = fma(-Lb[j * W + l], xr[l], s4[l & 3]);
-        xr[j] = gb[j] * ((s4[0] + s4[1]) + (s4[2] + s4[3]));
+        for (int k = j + 2; k < W; k++) cn[k] = Lt[(j + 1) * W + k];
+        gn = gb[j + 1];
+      }
+      if (j < m) {
+        const double xj = gc * xr[j];
+        xr[j] = xj;
+#pragma unroll
+        for (int kb = 0; kb < W; kb += 16) {
+          if (kb + 16 > j + 1 && kb < m) {
+#pragma unroll
+            for (int k = (kb > j + 1 ? kb : j + 1); k < kb + 16; k++) xr[k] = fma(-cc[k], xj, xr[k]);
+          }
+        }
       } else {
         xr[j] = 0.0;
       }
+#pragma unroll
+      for (int k = j + 2; k < W; k++) cc[k] = cn[k];
+      gc = gn;
     }
-    // back: y_j = x_j + gamma_j sum_{l>j} G_j[s_j][s_l] y_l  (zero coefficients beyond m)
+    // back: y_l final in descending l, then y_k += gamma_k N[k][l] y_l for k < l
 #pragma unroll
-    for (int j = W - 2; j >= 0; j--) {
-      if (j < m - 1) {
-        double s4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < W - 1; k++) cc[k] = Nt[(W - 1) * W + k];
 #pragma unroll
-        for (int lb = 0; lb < W; lb += 16) {
-          if (lb + 16 > j + 1 && lb < m) {
+    for (int l = W - 1; l >= 1; l--) {
 #pragma unroll
-            for (int l = (lb > j + 1 ? lb : j + 1); l < lb + 16; l++)
-              s4[l & 3] = fma(Nb[j * W + l], xr[l], s4[l & 3]);
+      for (int k = 0; k < l - 1; k++) cn[k] = Nt[(l - 1) * W + k];
+      if (l < m) {
+        const double yl = xr[l];
+#pragma unroll
+        for (int kb = 0; kb < W; kb += 16) {
+          if (kb < l && kb < m) {
+#pragma unroll
+            for (int k = kb; k < (kb + 16 < l ? kb + 16 : l); k++) xr[k] = fma(cc[k], yl, xr[k]);
           }
         }
-        xr[j] = fma(gb[j], (s4[0] + s4[1]) + (s4[2] + s4[3]), xr[j]);
       }
+#pragma unroll
+      for (int k = 0; k < l - 1; k++) cc[k] = cn[k];
     }
     // dense factor over window positions: Y[b][s_j][s_l] = y_j(l) (lane l holds column l)
     GT* Yb = a.Y + (size_t)b * W * W;
     if (lane < m) {
 #pragma unroll
       for (int j = 0; j < W; j++)
-        if (j < m) Yb[(size_t)sites[j] * W + site_r] = (GT)xr[j];
+        if (j < m) Yb[(size_t)sites[j] * W + sites[lane]] = (GT)xr[j];
     }
   }
   if (m > 0) {
     // zeros in the rows and columns of rejected sites (disjoint from the entries above)
-    const unsigned long long accm = __ballot(acc_r != 0);
 #pragma unroll
     for (int q = 0; q < FLV * W * W / 256; q++) {
       const int e = tid + 256 * q;
@@ -572,10 +836,6 @@ constexpr size_t apply_lds() {
   return sizeof(GT) * ((size_t)W * W + W * 64 + W * 65 + 64 * (W + 1));
 }
 
-size_t window_lds_bytes(int W, int flv) {
-  return (size_t)2 * flv * W * (W + 1) * 8 + (size_t)flv * W * 8 + (size_t)W * 4;
-}
-
 // 32 sites per window: measured fastest at n = 256 (per-flip work and the m x m solves
 // grow with W, launches with 1 / W); 16 and 64 are selectable through delay_depth.
 int window_size(int n, int flv) {
@@ -589,8 +849,8 @@ static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
   const int nt = (a.n + 63) / 64;
   const int tiles = a.Yp ? FLV * nt * nt : 0;
   if (a.do_window) {
-    const size_t lds = tiles && apply_lds<GT, W>() > window_lds<FLV, W>() ? apply_lds<GT, W>()
-                                                                           : window_lds<FLV, W>();
+    const size_t lds = tiles && apply_lds<GT, W>() > window_lds<GT, FLV, W>() ? apply_lds<GT, W>()
+                                                                           : window_lds<GT, FLV, W>();
     hipLaunchKernelGGL((window_kernel<GT, FLV, W>), dim3(1 + tiles), dim3(256), lds, s, a);
   } else if (tiles) {
     hipLaunchKernelGGL((window_apply_kernel<GT, W>), dim3(tiles), dim3(256), (apply_lds<GT, W>()), s, a);
