@@ -155,6 +155,73 @@ def measured_traffic(kernel_tag):
     return None
 
 
+def serve_leg(pkg, nreq, rate=20.0, seed=2024):
+    """Requests of uneven shape (square L in {4..12} or honeycomb L in {3..6}, beta in [1, 4],
+    U in [-8, 8], 2..10 sweeps) arriving as a Poisson process of `rate` per second, run through
+    the serving loop (admission by footprint, at most 16 chains at once).  A scaled-down
+    serving workload: the shapes and sweep counts are cut so that the leg takes seconds."""
+    from montecarlo_jl_amd import _lib as L
+    rng = np.random.default_rng(seed)
+    srv = L.Server(memory_budget=16 << 30, max_active=16)
+    t0 = time.perf_counter()
+    ids, sweeps, t_next = [], 0, 0.0
+    try:
+        for k in range(nreq):
+            t_next += rng.exponential(1.0 / rate)
+            while time.perf_counter() - t0 < t_next:
+                time.sleep(0.001)
+            if rng.random() < 0.5:
+                lat = pkg.SquareLattice(int(rng.choice([4, 6, 8, 10, 12])))
+            else:
+                lat = pkg.Honeycomb(int(rng.integers(3, 7)))
+            U = float(rng.uniform(-8.0, 8.0)) or 1.0
+            beta = float(rng.uniform(1.0, 4.0))
+            ns = int(rng.integers(2, 11))
+            model = pkg.HubbardModel(l=lat, U=U, mu=0.0)
+            par = pkg.DQMCParameters.create(beta=beta, delta_tau=0.1, safe_mult=10)
+            field = model.choose_field()(par, model)
+            field.rand(1000 + k)
+            ids.append(srv.submit(model.hopping_matrix(), field.conf, field.flavors, par.safe_mult,
+                                  par.delta_tau, field.alpha, 5000 + k, ns))
+            sweeps += ns
+        lats, queued = [], []
+        for rid in ids:
+            srv.wait(rid)
+            r = srv.result(rid)
+            lats.append(r["latency_s"])
+            queued.append(r["queued_s"])
+        wall = time.perf_counter() - t0
+        peak = srv.status()["peak_active"]
+    finally:
+        srv.close()
+    return {"requests": nreq, "arrival_rate_per_s": rate, "sweeps": sweeps, "wall_s": wall,
+            "aggregate_sweeps_per_s": sweeps / wall, "peak_active": peak,
+            "latency_ms": {"p50": float(np.percentile(lats, 50)) * 1e3,
+                           "p99": float(np.percentile(lats, 99)) * 1e3},
+            "queued_ms_p99": float(np.percentile(queued, 99)) * 1e3,
+            "workload": "square L in {4..12} or honeycomb L in {3..6}, beta U[1,4], U U[-8,8], "
+                        "2..10 sweeps per request, FP64; scaled down from the full serving mix"}
+
+
+def cpu_all_cores(procs, sweeps=1):
+    """The oracle on `procs` host processes at once, each an independent chain of the same
+    workload (1 sweep each): aggregate sweeps/s of the host.  Child processes of this script
+    (--cpu-child), which load the oracle only."""
+    import subprocess
+    t0 = time.perf_counter()
+    ps = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-child", str(sweeps)],
+                           stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+          for _ in range(procs)]
+    outs = [p.communicate()[0] for p in ps]
+    wall = time.perf_counter() - t0
+    secs = [json.loads(o.strip().splitlines()[-1])["seconds"] for o in outs if o.strip()]
+    if len(secs) != procs:
+        return None
+    return {"value": procs * sweeps / max(secs), "unit": "sweeps/s", "cores": procs,
+            "kind": "port", "sample": f"{procs} processes x {sweeps} sweep, each its own chain; "
+            f"slowest {max(secs):.1f} s, wall {wall:.1f} s"}
+
+
 def cpu_baseline(pkg, model, par, field, sweeps):
     """The CPU oracle (FP64 restatement of the reference algorithm), one core."""
     sys.path.insert(0, ROOT)
@@ -182,6 +249,11 @@ def main():
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--batch", type=int, default=8,
                     help="also time this many config-B chains swept together (0: skip)")
+    ap.add_argument("--serve", type=int, default=24,
+                    help="also run this many requests through the serving loop (0: skip)")
+    ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="host processes for the all-cores CPU baseline (0: skip)")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="short run for hardware-counter profiles: build the stack, then this many "
                          "(sweep_spatial + propagate) slice steps, and exit")
@@ -194,6 +266,12 @@ def main():
     pkg = load_pkg()
     model, par, field = config_b(pkg)
     n, flv, M = len(model.l), field.flavors, par.slices
+
+    if args.cpu_child > 0:
+        # one host process of the all-cores CPU baseline: the oracle only, no GPU
+        cps, cdt = cpu_baseline(pkg, model, par, field, args.cpu_child)
+        print(json.dumps({"sweeps": args.cpu_child, "seconds": cdt}))
+        return
 
     if args.profile_slices > 0:
         ch = make_chain(pkg, model, par, field, "fp64")
@@ -293,6 +371,11 @@ def main():
         for c in chains:
             c.close()
 
+    # ---- serving form: Poisson arrivals through the request loop -----------------------------
+    serving = None
+    if args.serve > 0:
+        serving = serve_leg(pkg, args.serve)
+
     # ---- CPU baseline --------------------------------------------------------------------
     cpu = None
     if not args.skip_cpu:
@@ -302,6 +385,16 @@ def main():
                          f"after the stack build, {cdt:.1f} s, oracle/dqmc_oracle.c on one host core",
                "host_cores_available": os.cpu_count(),
                "gpu_over_cpu": sps / cps}
+        if args.cpu_procs > 0:
+            cpu["all_cores"] = cpu_all_cores(args.cpu_procs)
+
+    # algorithmic work of a sweep: b n^3 (12 M + 46 K) + 2 b n^2 A (GEMM 2n^3, TRSM n^3,
+    # UDT 3n^3; A accepted flips per sweep), against the FP64 matrix-core peak
+    K = -(-M // par.safe_mult)
+    f_sweep = flv * n ** 3 * (12 * M + 46 * K) + 2 * flv * n ** 2 * (acc / args.steps)
+    sweep_roofline = {"flop_per_sweep": f_sweep, "achieved_tflops": f_sweep * sps / 1e12,
+                      "peak_tflops": PEAK["fp64_mfma_tflops"],
+                      "frac": f_sweep * sps / 1e12 / PEAK["fp64_mfma_tflops"]}
 
     out = {
         "metric": METRIC, "value": sps, "unit": "sweeps/s", "n_gpus": 1,
@@ -318,7 +411,9 @@ def main():
         "propagation_error_last": st["propagation_error_last"],
         "mixed": mixed,
         "batched": batched,
+        "serving": serving,
         "roofline": roofline,
+        "sweep_roofline": sweep_roofline,
         "kernels": kernels,
         "kernel_timing_region": {"sweeps": ksteps, "ms_per_sweep_with_events": t_events / ksteps * 1e3},
         "cpu_baseline": cpu,

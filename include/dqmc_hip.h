@@ -119,6 +119,48 @@ int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted);
  * update(::LocalSweep) (local_updates.jl:82) for independent chains. */
 int dqmc_batch_sweep(dqmc_chain_t* chains, int count, int nsweeps, int64_t* accepted);
 
+/* Serving form: a request loop over independent chains with continuous batching.  No
+ * reference counterpart as a library call: the reference serves one chain per Julia process
+ * through run! (DQMC.jl:144-394); a request is what one such process runs (create the stack,
+ * build it, nsweeps x update(::LocalSweep), read back field, G and statistics).
+ *   - Requests are admitted in arrival order while fewer than max_active chains run and the
+ *     sum of their footprints (dqmc_footprint_bytes) fits memory_budget; requests arriving
+ *     while others run are admitted as slots and memory free up, and run concurrently, each
+ *     on its own host thread and HIP stream on `device`.
+ *   - A result is the same as the chain run alone (its kernels touch only its own buffers).
+ *   - The server's entry points are called from one host thread; dqmc_server_last_error(s)
+ *     describes the last failure (s = NULL: the last creation failure of this thread).
+ *   - request_state: 0 queued, 1 running, 2 done, 3 failed.                              */
+typedef struct dqmc_server_s* dqmc_server_t;
+typedef struct {
+  int device;             /* HIP device ordinal of every chain                              */
+  int max_active;         /* chains running at once (>= 1)                                  */
+  int64_t memory_budget;  /* bytes of device memory the running chains may hold together    */
+} dqmc_server_config_t;
+typedef struct {
+  dqmc_config_t chain;    /* shape of the chain (chain.device is replaced by the server's)  */
+  const double* hopping;  /* n x n hopping matrix T (copied at submit)                       */
+  int ld_hopping;
+  const int8_t* field;    /* n x slices initial +-1 field (copied at submit)                 */
+  int ld_field;
+  uint64_t seed;          /* counter-stream seed, position 0 (dqmc_set_seed)                 */
+  int nsweeps;            /* sweeps after the stack build                                    */
+} dqmc_request_t;
+int dqmc_server_create(const dqmc_server_config_t* cfg, dqmc_server_t* out);
+int dqmc_server_submit(dqmc_server_t s, const dqmc_request_t* request, int64_t* id);
+int dqmc_server_poll(dqmc_server_t s, int64_t id, int* request_state);
+int dqmc_server_wait(dqmc_server_t s, int64_t id);
+/* Copies the final field (n x slices), G (n x flavors*n, physical = 0) and statistics of a
+ * finished request and releases it; latency[0] = submit -> done, latency[1] = time queued
+ * (seconds).  Any output pointer may be NULL.                                             */
+int dqmc_server_result(dqmc_server_t s, int64_t id, int8_t* field, int ld_field, double* G, int ld_G,
+                       int64_t* accepted, dqmc_stats_t* stats, double* latency);
+int dqmc_server_status(dqmc_server_t s, int* queued, int* active, int* peak_active,
+                       int64_t* used_bytes);
+/* Drops queued requests, waits for running ones, frees the server. */
+int dqmc_server_destroy(dqmc_server_t s);
+const char* dqmc_server_last_error(dqmc_server_t s);
+
 /* Equal-time observables on the device from the physical G (greens.jl:114-125):
  * occupation (measurements/constructors/occupation.jl:44-55), kinetic/total energy
  * (measurements/constructors/energy.jl:119-166), interaction energy with the reference's U

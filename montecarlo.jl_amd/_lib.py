@@ -27,6 +27,16 @@ class Config(C.Structure):
                 ("delay_depth", C.c_int), ("device", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_active", C.c_int), ("memory_budget", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("chain", Config), ("hopping", C.c_void_p), ("ld_hopping", C.c_int),
+                ("field", C.c_void_p), ("ld_field", C.c_int), ("seed", C.c_uint64),
+                ("nsweeps", C.c_int)]
+
+
 class Stats(C.Structure):
     _fields_ = [("accepted", C.c_int64), ("proposed", C.c_int64), ("uniforms_used", C.c_int64),
                 ("negative_probability", C.c_int64), ("ties", C.c_int64),
@@ -46,6 +56,8 @@ EXPORTED = [
     "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
+    "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
+    "dqmc_server_result", "dqmc_server_status", "dqmc_server_destroy", "dqmc_server_last_error",
 ]
 
 KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local", "local_apply", "udt_panel", "udt_update"]
@@ -98,6 +110,13 @@ def load():
         "dqmc_la_gemm": [i, i, i, _dp, _dp, _dp, i],
         "dqmc_set_kernel_timing": [vp, i],
         "dqmc_get_kernel_times": [vp, vp, vp, i],
+        "dqmc_server_create": [C.POINTER(ServerConfig), C.POINTER(vp)],
+        "dqmc_server_submit": [vp, C.POINTER(Request), C.POINTER(i64)],
+        "dqmc_server_poll": [vp, i64, C.POINTER(i)],
+        "dqmc_server_wait": [vp, i64],
+        "dqmc_server_result": [vp, i64, vp, i, vp, i, C.POINTER(i64), C.POINTER(Stats), vp],
+        "dqmc_server_status": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i64)],
+        "dqmc_server_destroy": [vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -105,6 +124,8 @@ def load():
         f.restype = C.c_int
     L.dqmc_last_error.argtypes = [vp]
     L.dqmc_last_error.restype = C.c_char_p
+    L.dqmc_server_last_error.argtypes = [vp]
+    L.dqmc_server_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -138,6 +159,82 @@ def batch_sweep(chains, nsweeps: int = 1) -> np.ndarray:
         raise DQMCError(f"dqmc_batch_sweep failed ({rc}): "
                         + "; ".join(m.decode() for m in msgs if m))
     return acc
+
+
+REQUEST_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
+class Server:
+    """Request loop over independent chains (dqmc_server_*): admission by HBM footprint,
+    at most `max_active` chains at once, continuous batching."""
+
+    def __init__(self, memory_budget: int, max_active: int = 8, device: int = 0):
+        L = load()
+        cfg = ServerConfig(device, max_active, memory_budget)
+        h = C.c_void_p()
+        rc = L.dqmc_server_create(C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise DQMCError(f"dqmc_server_create failed ({rc}): {L.dqmc_server_last_error(None).decode()}")
+        self._h = h
+        self._shapes = {}
+
+    def _c(self, rc, what):
+        if rc != 0:
+            raise DQMCError(f"{what} failed ({rc}): {load().dqmc_server_last_error(self._h).decode()}")
+
+    def submit(self, T, conf, flavors: int, safe_mult: int, delta_tau: float, alpha: float,
+               seed: int, nsweeps: int, precision: str = "fp64", delay_depth: int = 0) -> int:
+        T = _f(T)
+        conf = _f(conf, np.int8)
+        n, M = conf.shape
+        cfg = Config(n, flavors, M, safe_mult, delta_tau, alpha, PRECISION[precision], 1,
+                     delay_depth, 0)
+        rq = Request(cfg, T.ctypes.data, T.shape[0], conf.ctypes.data, n, seed, nsweeps)
+        rid = C.c_int64()
+        self._c(load().dqmc_server_submit(self._h, C.byref(rq), C.byref(rid)), "dqmc_server_submit")
+        self._shapes[rid.value] = (n, M, flavors)
+        return rid.value
+
+    def poll(self, rid: int) -> str:
+        st = C.c_int()
+        self._c(load().dqmc_server_poll(self._h, rid, C.byref(st)), "dqmc_server_poll")
+        return REQUEST_STATES[st.value]
+
+    def wait(self, rid: int):
+        self._c(load().dqmc_server_wait(self._h, rid), "dqmc_server_wait")
+
+    def result(self, rid: int) -> dict:
+        """Field, G (flavors, n, n), accepted flips, stats and latencies of a finished request."""
+        n, M, flv = self._shapes.pop(rid)
+        conf = np.zeros((n, M), dtype=np.int8, order="F")
+        G = np.zeros((n, flv * n), order="F")
+        acc = C.c_int64()
+        st = Stats()
+        lat = np.zeros(2)
+        self._c(load().dqmc_server_result(self._h, rid, conf.ctypes.data, n, G.ctypes.data, n,
+                                          C.byref(acc), C.byref(st), lat.ctypes.data),
+                "dqmc_server_result")
+        return {"field": conf, "greens": np.stack([G[:, b * n:(b + 1) * n] for b in range(flv)]),
+                "accepted": acc.value, "stats": {k: getattr(st, k) for k, _ in Stats._fields_},
+                "latency_s": float(lat[0]), "queued_s": float(lat[1])}
+
+    def status(self) -> dict:
+        q, a, p = C.c_int(), C.c_int(), C.c_int()
+        u = C.c_int64()
+        self._c(load().dqmc_server_status(self._h, C.byref(q), C.byref(a), C.byref(p), C.byref(u)),
+                "dqmc_server_status")
+        return {"queued": q.value, "active": a.value, "peak_active": p.value, "used_bytes": u.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dqmc_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HIPChain:

@@ -23,7 +23,7 @@
 using namespace dqmc;
 
 namespace {
-std::string g_create_error;
+thread_local std::string g_create_error;   // per host thread: chains may be created concurrently
 std::once_flag g_init_once;
 hipError_t g_init_status = hipSuccess;
 

@@ -1,0 +1,269 @@
+// Serving form of the sweep: a request loop that admits independent chains by their HBM
+// footprint and runs them concurrently on one device (continuous batching).
+//
+// The reference runs one chain per Julia process through run! (DQMC.jl:144-394); this is
+// the many-request form of that loop.  A request carries a chain's shape, its hopping
+// matrix, initial field, counter-stream seed and sweep count.  A scheduler thread admits
+// queued requests in arrival order while the number of running chains is below
+// max_active and the sum of their footprints (dqmc_footprint_bytes: the UDT stack and
+// every other device buffer of the chain) fits the memory budget.  Each admitted request
+// runs on its own host thread and HIP stream: create, set hopping / field / seed,
+// build_stack, sweep, read back field, G and statistics, destroy.  Requests that arrive
+// while others run are admitted as soon as a slot and memory free up, so their kernels
+// overlap with the running ones.  A chain's result does not depend on what else runs:
+// every kernel of a chain reads and writes only that chain's buffers, in its stream order.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dqmc_hip.h"
+
+namespace {
+
+enum { kQueued = 0, kRunning = 1, kDone = 2, kFailed = 3 };
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  int64_t id = 0;
+  dqmc_config_t cfg{};
+  std::vector<double> T;
+  std::vector<int8_t> conf;
+  uint64_t seed = 0;
+  int nsweeps = 0;
+  int64_t footprint = 0;
+  int state = kQueued;            // guarded by the server mutex, as are t_start and t_done
+  std::string err;
+  std::vector<int8_t> conf_out;
+  std::vector<double> G_out;
+  dqmc_stats_t stats{};
+  int64_t accepted = 0;
+  double t_submit = 0, t_start = 0, t_done = 0;
+};
+
+}  // namespace
+
+struct dqmc_server_s {
+  dqmc_server_config_t cfg{};
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::shared_ptr<Request>> queue;
+  std::unordered_map<int64_t, std::shared_ptr<Request>> reqs;
+  int64_t next_id = 1;
+  int64_t used_bytes = 0;
+  int active = 0;
+  int peak_active = 0;
+  bool stop = false;
+  std::thread scheduler;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string t_server_error;
+
+// One request, start to finish, on the calling (worker) thread; false on failure (r.err).
+bool run_request(Request& r, int device) {
+  dqmc_config_t c = r.cfg;
+  c.device = device;
+  dqmc_chain_t h = nullptr;
+  auto fail = [&](const char* what) {
+    r.err = std::string(what) + ": " + dqmc_last_error(h);
+    return false;
+  };
+  auto body = [&]() -> bool {
+    if (dqmc_create(&c, &h) != DQMC_OK) return fail("dqmc_create");
+    const int n = c.n_sites;
+    if (dqmc_set_hopping(h, r.T.data(), n) != DQMC_OK) return fail("dqmc_set_hopping");
+    if (dqmc_set_field(h, r.conf.data(), n) != DQMC_OK) return fail("dqmc_set_field");
+    if (dqmc_set_seed(h, r.seed, 0) != DQMC_OK) return fail("dqmc_set_seed");
+    if (dqmc_build_stack(h) != DQMC_OK) return fail("dqmc_build_stack");
+    if (dqmc_sweep(h, r.nsweeps, &r.accepted) != DQMC_OK) return fail("dqmc_sweep");
+    r.conf_out.assign((size_t)n * c.slices, 0);
+    if (dqmc_get_field(h, r.conf_out.data(), n) != DQMC_OK) return fail("dqmc_get_field");
+    r.G_out.assign((size_t)n * n * c.flavors, 0.0);
+    if (dqmc_get_greens(h, r.G_out.data(), n, 0) != DQMC_OK) return fail("dqmc_get_greens");
+    if (dqmc_get_stats(h, &r.stats) != DQMC_OK) return fail("dqmc_get_stats");
+    return true;
+  };
+  const bool ok = body();
+  if (h) dqmc_destroy(h);
+  return ok;
+}
+
+void scheduler_loop(dqmc_server_s* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    s->cv.wait(lk, [&]() {
+      if (s->stop) return true;
+      if (s->queue.empty()) return false;
+      const auto& r = s->queue.front();
+      return s->active < s->cfg.max_active && s->used_bytes + r->footprint <= s->cfg.memory_budget;
+    });
+    if (s->stop) return;
+    std::shared_ptr<Request> r = s->queue.front();
+    s->queue.pop_front();
+    s->active++;
+    if (s->active > s->peak_active) s->peak_active = s->active;
+    s->used_bytes += r->footprint;
+    r->state = kRunning;
+    r->t_start = now_s();
+    const int device = s->cfg.device;
+    std::thread([s, r, device]() {
+      const bool ok = run_request(*r, device);
+      std::lock_guard<std::mutex> g(s->mu);
+      r->state = ok ? kDone : kFailed;
+      r->t_done = now_s();
+      s->active--;
+      s->used_bytes -= r->footprint;
+      s->cv.notify_all();
+    }).detach();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dqmc_server_last_error(dqmc_server_t s) {
+  return s ? s->err.c_str() : t_server_error.c_str();
+}
+
+int dqmc_server_create(const dqmc_server_config_t* cfg, dqmc_server_t* out) {
+  if (!cfg || !out) { t_server_error = "dqmc_server_create: null argument"; return DQMC_E_INVALID; }
+  if (cfg->max_active < 1 || cfg->memory_budget <= 0) {
+    t_server_error = "dqmc_server_create: max_active >= 1 and memory_budget > 0 required";
+    return DQMC_E_INVALID;
+  }
+  auto* s = new dqmc_server_s();
+  s->cfg = *cfg;
+  s->scheduler = std::thread(scheduler_loop, s);
+  *out = s;
+  return DQMC_OK;
+}
+
+int dqmc_server_submit(dqmc_server_t s, const dqmc_request_t* rq, int64_t* id) {
+  if (!s || !rq || !id) return DQMC_E_INVALID;
+  const dqmc_config_t& c = rq->chain;
+  int64_t fp = 0;
+  if (dqmc_footprint_bytes(&c, &fp) != DQMC_OK) {
+    s->err = std::string("dqmc_server_submit: ") + dqmc_last_error(nullptr);
+    return DQMC_E_INVALID;
+  }
+  const int n = c.n_sites;
+  if (!rq->hopping || !rq->field || rq->ld_hopping < n || rq->ld_field < n || rq->nsweeps < 0) {
+    s->err = "dqmc_server_submit: null hopping/field, ld < n_sites or nsweeps < 0";
+    return DQMC_E_INVALID;
+  }
+  if (fp > s->cfg.memory_budget) {
+    s->err = "dqmc_server_submit: the chain's footprint exceeds the server's memory budget";
+    return DQMC_E_ALLOC;
+  }
+  auto r = std::make_shared<Request>();
+  r->cfg = c;
+  r->T.resize((size_t)n * n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) r->T[(size_t)i + (size_t)j * n] = rq->hopping[(size_t)i + (size_t)j * rq->ld_hopping];
+  r->conf.resize((size_t)n * c.slices);
+  for (int l = 0; l < c.slices; l++)
+    for (int i = 0; i < n; i++) r->conf[(size_t)i + (size_t)l * n] = rq->field[(size_t)i + (size_t)l * rq->ld_field];
+  r->seed = rq->seed;
+  r->nsweeps = rq->nsweeps;
+  r->footprint = fp;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (s->stop) { s->err = "dqmc_server_submit: server is shutting down"; return DQMC_E_STATE; }
+  r->id = s->next_id++;
+  r->t_submit = now_s();
+  s->reqs[r->id] = r;
+  s->queue.push_back(r);
+  s->cv.notify_all();
+  *id = r->id;
+  return DQMC_OK;
+}
+
+int dqmc_server_poll(dqmc_server_t s, int64_t id, int* state) {
+  if (!s || !state) return DQMC_E_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) { s->err = "dqmc_server_poll: unknown request id"; return DQMC_E_INVALID; }
+  *state = it->second->state;
+  return DQMC_OK;
+}
+
+int dqmc_server_wait(dqmc_server_t s, int64_t id) {
+  if (!s) return DQMC_E_INVALID;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) { s->err = "dqmc_server_wait: unknown request id"; return DQMC_E_INVALID; }
+  std::shared_ptr<Request> r = it->second;
+  s->cv.wait(lk, [&]() { return r->t_done > 0; });
+  return DQMC_OK;
+}
+
+int dqmc_server_result(dqmc_server_t s, int64_t id, int8_t* conf, int ld_conf, double* G, int ld_G,
+                       int64_t* accepted, dqmc_stats_t* stats, double* latency) {
+  if (!s) return DQMC_E_INVALID;
+  std::shared_ptr<Request> r;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) { s->err = "dqmc_server_result: unknown request id"; return DQMC_E_INVALID; }
+    r = it->second;
+    if (r->t_done <= 0) { s->err = "dqmc_server_result: request not finished"; return DQMC_E_STATE; }
+    s->reqs.erase(it);
+  }
+  if (r->state == kFailed) { s->err = r->err; return DQMC_E_STATE; }
+  const int n = r->cfg.n_sites;
+  if (conf) {
+    if (ld_conf < n) { s->err = "dqmc_server_result: ld_conf < n_sites"; return DQMC_E_INVALID; }
+    for (int l = 0; l < r->cfg.slices; l++)
+      for (int i = 0; i < n; i++) conf[(size_t)i + (size_t)l * ld_conf] = r->conf_out[(size_t)i + (size_t)l * n];
+  }
+  if (G) {
+    if (ld_G < n) { s->err = "dqmc_server_result: ld_G < n_sites"; return DQMC_E_INVALID; }
+    for (int j = 0; j < n * r->cfg.flavors; j++)
+      for (int i = 0; i < n; i++) G[(size_t)i + (size_t)j * ld_G] = r->G_out[(size_t)i + (size_t)j * n];
+  }
+  if (accepted) *accepted = r->accepted;
+  if (stats) *stats = r->stats;
+  if (latency) {
+    latency[0] = r->t_done - r->t_submit;   // submit -> result ready
+    latency[1] = r->t_start - r->t_submit;  // time queued before admission
+  }
+  return DQMC_OK;
+}
+
+int dqmc_server_status(dqmc_server_t s, int* queued, int* active, int* peak_active, int64_t* used_bytes) {
+  if (!s) return DQMC_E_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (queued) *queued = (int)s->queue.size();
+  if (active) *active = s->active;
+  if (peak_active) *peak_active = s->peak_active;
+  if (used_bytes) *used_bytes = s->used_bytes;
+  return DQMC_OK;
+}
+
+int dqmc_server_destroy(dqmc_server_t s) {
+  if (!s) return DQMC_E_INVALID;
+  {
+    std::unique_lock<std::mutex> lk(s->mu);
+    // queued requests are dropped; running ones finish first
+    s->queue.clear();
+    s->stop = true;
+    s->cv.notify_all();
+    s->cv.wait(lk, [&]() { return s->active == 0; });
+  }
+  s->scheduler.join();
+  delete s;
+  return DQMC_OK;
+}
+
+}  // extern "C"
