@@ -107,11 +107,12 @@ __device__ __forceinline__ int vpos(int c) { return (c & 3) * (W / 4) + (c >> 2)
 // the block Bn (over Uw) and, for W <= 32, the staged operands and FP32 Z scratch after it.
 template <typename GT, int FLV, int W>
 constexpr size_t window_lds() {
-  const size_t loop = (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4 + 16;
+  const size_t loop = (size_t)2 * FLV * W * (W + 1) * 8 + (size_t)FLV * W * 8 + (size_t)W * 4 + 16 +
+                      (W <= 32 ? (size_t)FLV * W * (W + 1) * 8 : 0);   // + u^T (W <= 32)
   const size_t block = W > 32 ? 0
                               : (size_t)FLV * W * (W + 1) * 8 + (size_t)4 * FLV * W * (W + 1) * sizeof(GT) +
                                     (sizeof(GT) == 4 ? (size_t)4 * W * 17 * 4 : 0);
-  return loop > block ? loop : block;
+  return (loop > block ? loop : block) + 16;     // + the two progress flags (last 16 bytes)
 }
 
 // The window's block of G after the previous window's flips, from the source copy:
@@ -323,9 +324,17 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   double* gam = Vw + FLV * W * LD;                        // [b][l]
   int* sites = reinterpret_cast<int*>(gam + FLV * W);     // [l], window index of flip l
   unsigned long long* shm = reinterpret_cast<unsigned long long*>(sites + W);   // m, accepted mask
+  double* UwT = reinterpret_cast<double*>(shm + 2);       // W <= 32: u^T[b][r][flip], stride W + 1
   const int n = a.n, w0 = a.w0;
   const int Wn = n - w0 < W ? n - w0 : W;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // progress of the decision wave for the waves that build the factor beside it:
+  // flags[0] = flips published, flags[1] = final flip count (-1 while deciding)
+  int* flags = reinterpret_cast<int*>(smem + window_lds<GT, FLV, W>() - 16);
+  if (tid == 0) {
+    flags[0] = 0;
+    flags[1] = -1;
+  }
   const long long pos0 = a.ctr->u_pos;
   const long long log0 = a.ctr->log_len;
   const int lr = lane < Wn ? lane : Wn - 1;
@@ -359,6 +368,17 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   double p_r = 0.0, gam0_r = 0.0, gam1_r = 0.0;
 
   if constexpr (W <= 32) {
+    // rows and columns not yet published read as exact zeros in the factor waves
+    for (int e = tid; e < FLV * W * LD; e += 256) {
+      Vw[e] = 0.0;
+      UwT[e] = 0.0;
+    }
+    __syncthreads();
+  }
+  // factor waves (W <= 32): wave 1 column r of X, wave 2 row r of (I - N)^-1, and gammas
+  typedef double fvec __attribute__((ext_vector_type(16)));
+  fvec fx_lo = 0.0, fx_hi = 0.0, gc_lo = 0.0, gc_hi = 0.0;
+  if constexpr (W <= 32) {
     // Single-wave decisions: wave 0 lane 32 f + r holds row r of flavour block f, all W
     // columns in registers; u of an accepted flip at site t is each lane's own G[r][t],
     // and row t reaches the other lanes through LDS inside the wave (no workgroup barrier).
@@ -368,6 +388,7 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
     const int f = lane >> 5, r = lane & 31;
     const bool fok = f < FLV;
     const int fb = fok ? f : 0;          // lanes of an absent second block read block 0, unused
+    constexpr int UT = LD;               // row stride of u^T (fits the Uw storage)
     if (wave == 0) {
       const bool rok = fok && r < Wn;
       const int rc = r < Wn ? r : Wn - 1;
@@ -423,7 +444,7 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
           }
         }
         const double uraw = (r == t ? 1.0 : 0.0) - gt;                     // u_t(r), unscaled
-        if (fok && r < W) Uw[(size_t)(f * W + m) * LD + r] = uraw;
+        if (fok && r < W) UwT[(size_t)(f * W + r) * UT + m] = uraw;         // u^T: [f][r][flip]
         site_r = lane == m ? t : site_r;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_s_waitcnt(0xC07F);
@@ -444,8 +465,15 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
           }
         }
         dg = fma(-ug, Vr[r < W ? r : W - 1], dg);
+        if (lane == 0) sites[m] = t;
         m++;
+        // flip m - 1 (u, row, gamma, site; written by several lanes) is complete: publish it
+        // to the factor waves
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(flags, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) __hip_atomic_store(flags + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       const unsigned long long am = __ballot(acc_r != 0);    // whole wave active here
       if (lane == 0) {
         shm[0] = (unsigned long long)m;
@@ -463,6 +491,68 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
           if (a.p_log) a.p_log[log0 + lane] = p_r;
         }
         if (acc_r) a.conf[w0 + lane] = (int8_t)(-cfl);
+      }
+    } else if (wave == 1 || wave == 2) {
+      // Beside the decisions, flip by flip as wave 0 publishes them (lane 32 f + r, block f):
+      //   wave 1: row k of X = (I + Gamma L)^-1 Gamma,
+      //           x_k = gamma_k (e_k - sum_{l<k} u_l(s_k) x_l); lane r holds column r of X;
+      //   wave 2: column k of (I - N)^-1, N[i][k] = gamma_i G_i[s_i][s_k] (i < k), by bordering:
+      //           Binv[:, k] = Binv_(k) N[:, k], Binv[k][k] = 1; lane r holds row r.
+      // Flip k needs only flips < k and its own site and gamma, all published before it.
+      // The wait is bounded, so the wave always leaves the loop.
+      int k = 0;
+      long spins = 0;
+      for (;;) {
+        int avail = __hip_atomic_load(flags, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (avail <= k) {
+          const int fin = __hip_atomic_load(flags + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (fin < 0) {
+            if (++spins > (1l << 22)) break;
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+          }
+          avail = __hip_atomic_load(flags, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (avail <= k) break;
+        }
+        const int sk = sites[k];
+        const double gk = gam[fb * W + k];
+        const int kk = k & 15;
+        double cf[W];
+        if (wave == 1) {
+          // L[k][l] = u_l(s_k): row s_k of u^T, contiguous
+          const double* ur = UwT + (size_t)(fb * W + sk) * UT;    // zero beyond flip k - 1
+#pragma unroll
+          for (int l = 0; l < W; l++) cf[l] = ur[l];
+          double s4[4] = {r == k ? 1.0 : 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int l = 0; l < W; l++) {
+            const double xl = l < 16 ? fx_lo[l & 15] : fx_hi[l & 15];
+            s4[l & 3] = fma(-cf[l], xl, s4[l & 3]);
+          }
+          const double xk = gk * ((s4[0] + s4[1]) + (s4[2] + s4[3]));
+          const double olo = fx_lo[kk], ohi = fx_hi[kk];
+          fx_lo[kk] = k < 16 ? xk : olo;
+          fx_hi[kk] = k < 16 ? ohi : xk;
+        } else {
+          // N[i][k] / gamma_i = G_i[s_i][s_k]: column s_k of the published rows
+#pragma unroll
+          for (int i = 0; i < W; i++) cf[i] = Vw[(size_t)(fb * W + i) * W + sk];
+          const double glo = gc_lo[kk], ghi = gc_hi[kk];
+          gc_lo[kk] = k < 16 ? gk : glo;
+          gc_hi[kk] = k < 16 ? ghi : gk;
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int i = 0; i < W; i++) {
+            const double gi = i < 16 ? gc_lo[i & 15] : gc_hi[i & 15];
+            const double bi = i < 16 ? fx_lo[i & 15] : fx_hi[i & 15];
+            s4[i & 3] = fma(bi, gi * cf[i], s4[i & 3]);          // zeros beyond flip k - 1
+          }
+          const double yk = r == k ? 1.0 : (s4[0] + s4[1]) + (s4[2] + s4[3]);
+          const double olo = fx_lo[kk], ohi = fx_hi[kk];
+          fx_lo[kk] = k < 16 ? yk : olo;
+          fx_hi[kk] = k < 16 ? ohi : yk;
+        }
+        k++;
       }
     }
     __syncthreads();
@@ -575,99 +665,134 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   }
   const unsigned long long accm = shm[1];    // accepted window positions
 
-  // Gather the m x m coefficient blocks, transposed, over the (now free) Uw/Vw storage:
-  //   Lt[b][l][j] = u_l(s_j) (l < j),  Nt[b][l][j] = gamma_j G_j[s_j][s_l] (j < l),
-  // zero elsewhere: column l of L and of Gamma N is a contiguous row.
-  double* Lc = Uw;                 // Lt[b] at 2 b W^2, Nt[b] at (2 b + 1) W^2
-  if (m > 0) {
-    constexpr int E = 2 * FLV * W * W / 256;
-    double gv[E];
+  if constexpr (W <= 32) {
+    // Y = (I - N)^-1 X on the matrix cores: X (wave 1) and (I - N)^-1 (wave 2) through LDS
+    // over the free Uw / Vw storage, then scattered to the window positions.
+    if (m > 0) {
+      double* Xs = Uw;                  // [b][j][c], leading dimension LD
+      double* Bs = Vw;                  // [b][i][j] = Binv[j][i] (transposed), leading dimension LD
+      const int f = lane >> 5, r = lane & 31;
+      if ((wave == 1 || wave == 2) && f < FLV && r < W) {
+        double* dst = (wave == 1 ? Xs : Bs) + (size_t)f * W * LD;
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-      const int e = tid + 256 * q;
-      const int mat = e / (W * W), rem = e - mat * W * W;
-      const int l = rem / W, j = rem - l * W;
-      const int b = mat >> 1;
-      // unconditional in-range loads (sites[] is 0 beyond m), then the mask
-      const double vl = Uw[(b * W + l) * LD + sites[j]];
-      const double vn = W <= 32 ? Vw[(b * W + j) * W + sites[l]] : Vw[(b * W + j) * LD + vpos<W>(sites[l])];
-      const double gj = gam[b * W + j];
-      const bool isl = (mat & 1) == 0;
-      const bool keep = isl ? (l < j && j < m) : (j < l && l < m);
-      gv[q] = keep ? (isl ? vl : gj * vn) : 0.0;
+        for (int j = 0; j < W; j++) dst[j * LD + r] = j < 16 ? fx_lo[j & 15] : fx_hi[j & 15];
+      }
+      __syncthreads();
+      constexpr int CT = W / 16, KS = W / 4;
+      const int li = lane & 15, lk = lane >> 4;
+      for (int unit = wave; unit < FLV * CT * CT; unit += 4) {
+        const int b = unit / (CT * CT), t2 = unit - b * CT * CT;
+        const int ti = t2 / CT, ci = t2 - ti * CT;
+        const double* Bb = Bs + (size_t)b * W * LD;
+        const double* Xb = Xs + (size_t)b * W * LD;
+        dx4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s4 = 0; s4 < KS; s4++)
+          acc = mfma16(Bb[(4 * s4 + lk) * LD + ti * 16 + li], Xb[(4 * s4 + lk) * LD + ci * 16 + li], acc);
+        GT* Yb = a.Y + (size_t)b * W * W;
+        const int c = ci * 16 + li;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int j = ti * 16 + mfma_row<double>(lane, q);
+          if (j < m && c < m) Yb[(size_t)sites[j] * W + sites[c]] = (GT)acc[q];
+        }
+      }
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < E; q++) Lc[tid + 256 * q] = gv[q];
-    __syncthreads();
-  }
+  } else {
+    // Gather the m x m coefficient blocks, transposed, over the (now free) Uw/Vw storage:
+    //   Lt[b][l][j] = u_l(s_j) (l < j),  Nt[b][l][j] = gamma_j G_j[s_j][s_l] (j < l),
+    // zero elsewhere: column l of L and of Gamma N is a contiguous row.
+    double* Lc = Uw;                 // Lt[b] at 2 b W^2, Nt[b] at (2 b + 1) W^2
+    if (m > 0) {
+      constexpr int E = 2 * FLV * W * W / 256;
+      double gv[E];
+  #pragma unroll
+      for (int q = 0; q < E; q++) {
+        const int e = tid + 256 * q;
+        const int mat = e / (W * W), rem = e - mat * W * W;
+        const int l = rem / W, j = rem - l * W;
+        const int b = mat >> 1;
+        // unconditional in-range loads (sites[] is 0 beyond m), then the mask
+        const double vl = Uw[(b * W + l) * LD + sites[j]];
+        const double vn = W <= 32 ? Vw[(b * W + j) * W + sites[l]] : Vw[(b * W + j) * LD + vpos<W>(sites[l])];
+        const double gj = gam[b * W + j];
+        const bool isl = (mat & 1) == 0;
+        const bool keep = isl ? (l < j && j < m) : (j < l && l < m);
+        gv[q] = keep ? (isl ? vl : gj * vn) : 0.0;
+      }
+      __syncthreads();
+  #pragma unroll
+      for (int q = 0; q < E; q++) Lc[tid + 256 * q] = gv[q];
+      __syncthreads();
+    }
 
-  // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma, wave b for flavour block b, lane = column c,
-  // column c of Y in registers.  Column-oriented substitutions: once x_j (y_l) is final it
-  // is subtracted from (added to) the remaining rows, so each step's own chain is one
-  // multiply and one fma; the next column of coefficients is loaded during the step.
-  if (wave < FLV && m > 0) {
-    const int b = wave;
-    const double* Lt = Lc + 2 * b * W * W;
-    const double* Nt = Lc + (2 * b + 1) * W * W;
-    const double* gb = gam + b * W;
-    double xr[W], cc[W], cn[W] = {};
-#pragma unroll
-    for (int k = 0; k < W; k++) xr[k] = lane == k ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 1; k < W; k++) cc[k] = Lt[k];
-    double gc = gb[0], gn = 0.0;
-    // forward: x_j = gamma_j s_j, then s_k -= L[k][j] x_j for k > j
-#pragma unroll
-    for (int j = 0; j < W; j++) {
-      if (j + 1 < W) {                   // unconditional: columns beyond m are zero
-#pragma unroll
-        for (int k = j + 2; k < W; k++) cn[k] = Lt[(j + 1) * W + k];
-        gn = gb[j + 1];
+    // Y = (I - N)^-1 (I + Gamma L)^-1 Gamma, wave b for flavour block b, lane = column c,
+    // column c of Y in registers.  Column-oriented substitutions: once x_j (y_l) is final it
+    // is subtracted from (added to) the remaining rows, so each step's own chain is one
+    // multiply and one fma; the next column of coefficients is loaded during the step.
+    if (wave < FLV && m > 0) {
+      const int b = wave;
+      const double* Lt = Lc + 2 * b * W * W;
+      const double* Nt = Lc + (2 * b + 1) * W * W;
+      const double* gb = gam + b * W;
+      double xr[W], cc[W], cn[W] = {};
+  #pragma unroll
+      for (int k = 0; k < W; k++) xr[k] = lane == k ? 1.0 : 0.0;
+  #pragma unroll
+      for (int k = 1; k < W; k++) cc[k] = Lt[k];
+      double gc = gb[0], gn = 0.0;
+      // forward: x_j = gamma_j s_j, then s_k -= L[k][j] x_j for k > j
+  #pragma unroll
+      for (int j = 0; j < W; j++) {
+        if (j + 1 < W) {                   // unconditional: columns beyond m are zero
+  #pragma unroll
+          for (int k = j + 2; k < W; k++) cn[k] = Lt[(j + 1) * W + k];
+          gn = gb[j + 1];
+        }
+        if (j < m) {
+          const double xj = gc * xr[j];
+          xr[j] = xj;
+  #pragma unroll
+          for (int kb = 0; kb < W; kb += 16) {
+            if (kb + 16 > j + 1 && kb < m) {
+  #pragma unroll
+              for (int k = (kb > j + 1 ? kb : j + 1); k < kb + 16; k++) xr[k] = fma(-cc[k], xj, xr[k]);
+            }
+          }
+        } else {
+          xr[j] = 0.0;
+        }
+  #pragma unroll
+        for (int k = j + 2; k < W; k++) cc[k] = cn[k];
+        gc = gn;
       }
-      if (j < m) {
-        const double xj = gc * xr[j];
-        xr[j] = xj;
-#pragma unroll
-        for (int kb = 0; kb < W; kb += 16) {
-          if (kb + 16 > j + 1 && kb < m) {
-#pragma unroll
-            for (int k = (kb > j + 1 ? kb : j + 1); k < kb + 16; k++) xr[k] = fma(-cc[k], xj, xr[k]);
+      // back: y_l final in descending l, then y_k += gamma_k N[k][l] y_l for k < l
+  #pragma unroll
+      for (int k = 0; k < W - 1; k++) cc[k] = Nt[(W - 1) * W + k];
+  #pragma unroll
+      for (int l = W - 1; l >= 1; l--) {
+  #pragma unroll
+        for (int k = 0; k < l - 1; k++) cn[k] = Nt[(l - 1) * W + k];
+        if (l < m) {
+          const double yl = xr[l];
+  #pragma unroll
+          for (int kb = 0; kb < W; kb += 16) {
+            if (kb < l && kb < m) {
+  #pragma unroll
+              for (int k = kb; k < (kb + 16 < l ? kb + 16 : l); k++) xr[k] = fma(cc[k], yl, xr[k]);
+            }
           }
         }
-      } else {
-        xr[j] = 0.0;
+  #pragma unroll
+        for (int k = 0; k < l - 1; k++) cc[k] = cn[k];
       }
-#pragma unroll
-      for (int k = j + 2; k < W; k++) cc[k] = cn[k];
-      gc = gn;
-    }
-    // back: y_l final in descending l, then y_k += gamma_k N[k][l] y_l for k < l
-#pragma unroll
-    for (int k = 0; k < W - 1; k++) cc[k] = Nt[(W - 1) * W + k];
-#pragma unroll
-    for (int l = W - 1; l >= 1; l--) {
-#pragma unroll
-      for (int k = 0; k < l - 1; k++) cn[k] = Nt[(l - 1) * W + k];
-      if (l < m) {
-        const double yl = xr[l];
-#pragma unroll
-        for (int kb = 0; kb < W; kb += 16) {
-          if (kb < l && kb < m) {
-#pragma unroll
-            for (int k = kb; k < (kb + 16 < l ? kb + 16 : l); k++) xr[k] = fma(cc[k], yl, xr[k]);
-          }
-        }
+      // dense factor over window positions: Y[b][s_j][s_l] = y_j(l) (lane l holds column l)
+      GT* Yb = a.Y + (size_t)b * W * W;
+      if (lane < m) {
+  #pragma unroll
+        for (int j = 0; j < W; j++)
+          if (j < m) Yb[(size_t)sites[j] * W + sites[lane]] = (GT)xr[j];
       }
-#pragma unroll
-      for (int k = 0; k < l - 1; k++) cc[k] = cn[k];
-    }
-    // dense factor over window positions: Y[b][s_j][s_l] = y_j(l) (lane l holds column l)
-    GT* Yb = a.Y + (size_t)b * W * W;
-    if (lane < m) {
-#pragma unroll
-      for (int j = 0; j < W; j++)
-        if (j < m) Yb[(size_t)sites[j] * W + sites[lane]] = (GT)xr[j];
     }
   }
   if (m > 0) {
