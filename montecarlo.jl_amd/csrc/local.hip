@@ -393,21 +393,21 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       const bool rok = fok && r < Wn;
       const int rc = r < Wn ? r : Wn - 1;
       const double* Bf = Bn + (size_t)fb * W * LD;
+      // every read in flight at once: in-range (clamped) addresses, masked by an exact
+      // multiplication with 0 or 1 (the block is finite), no per-column branch
       rowvec lo, hi;
 #pragma unroll
       for (int c = 0; c < 16; c++) {
         const int cl = c < Wn ? c : Wn - 1;
-        const double xl = Bf[cl * LD + rc];
-        lo[c] = (rok && c < Wn) ? xl : 0.0;
+        lo[c] = Bf[cl * LD + rc] * ((rok && c < Wn) ? 1.0 : 0.0);
         if constexpr (W > 16) {
           const int ch = c + 16 < Wn ? c + 16 : Wn - 1;
-          const double xh = Bf[ch * LD + rc];
-          hi[c] = (rok && c + 16 < Wn) ? xh : 0.0;
+          hi[c] = Bf[ch * LD + rc] * ((rok && c + 16 < Wn) ? 1.0 : 0.0);
         } else {
           hi[c] = 0.0;
         }
       }
-      double dg = rok ? Bf[rc * LD + rc] : 0.0;
+      double dg = Bf[rc * LD + rc] * (rok ? 1.0 : 0.0);
       // order the reads of Bn before the first flip's stores over it
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_s_waitcnt(0xC07F);
