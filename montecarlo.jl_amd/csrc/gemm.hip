@@ -73,6 +73,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   };
   // two register sets (chunks of even / odd index), so that two chunks can be in flight
   T ra[kLoadsPerThread], rb[kLoadsPerThread];
+  // A chunk's loads are all issued before any is used: in-range (clamped) addresses, then
+  // the out-of-range elements zeroed; the middle diagonal (a uniform choice per launch) is
+  // gathered for the chunk's k values in a second batch of loads.
   auto load = [&](int c, T* reg) {
     const int k0 = c * kKC;
 #pragma unroll
@@ -81,18 +84,50 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
       coord(q, mn, kk);
       const int gk = k0 + kk;
       const int gkc = gk < K ? gk : K - 1;
-      T v;
       if (q < 16) {
         const int gm = m0 + mn, gmc = gm < M ? gm : M - 1;
-        v = TA == 0 ? A[(size_t)gmc + (size_t)gkc * lda] : A[(size_t)gkc + (size_t)gmc * lda];
-        if (has_mid) v = T(v * diag_at(it.dmid, gkc));
-        v = (gm < M && gk < K) ? v : T(0);
+        reg[q] = TA == 0 ? A[(size_t)gmc + (size_t)gkc * lda] : A[(size_t)gkc + (size_t)gmc * lda];
       } else {
         const int gn = n0 + mn, gnc = gn < N ? gn : N - 1;
-        v = TB == 0 ? B[(size_t)gkc + (size_t)gnc * ldb] : B[(size_t)gnc + (size_t)gkc * ldb];
-        v = (gn < N && gk < K) ? v : T(0);
+        reg[q] = TB == 0 ? B[(size_t)gkc + (size_t)gnc * ldb] : B[(size_t)gnc + (size_t)gkc * ldb];
       }
-      reg[q] = v;
+    }
+    if (has_mid) {
+      double f[16];
+      if (it.dmid.vec) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          int mn, kk;
+          coord(q, mn, kk);
+          const int gk = k0 + kk;
+          f[q] = it.dmid.vec[gk < K ? gk : K - 1];
+        }
+        if (it.dmid.invert) {
+#pragma unroll
+          for (int q = 0; q < 16; q++) f[q] = 1.0 / f[q];
+        }
+      } else {
+        int8_t cf[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          int mn, kk;
+          coord(q, mn, kk);
+          const int gk = k0 + kk;
+          cf[q] = it.dmid.conf[gk < K ? gk : K - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) f[q] = cf[q] > 0 ? it.dmid.pos : it.dmid.neg;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) reg[q] = T(reg[q] * f[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < kLoadsPerThread; q++) {
+      int mn, kk;
+      coord(q, mn, kk);
+      const int gk = k0 + kk;
+      const bool ok = q < 16 ? (m0 + mn < M && gk < K) : (n0 + mn < N && gk < K);
+      reg[q] = ok ? reg[q] : T(0);
     }
   };
   auto store = [&](int buf, const T* reg) {
