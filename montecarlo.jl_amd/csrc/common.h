@@ -94,6 +94,8 @@ hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s);
 hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s);
 hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode, hipStream_t s);
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s);
+// Q accumulation into U (U preset to I) for every panel, in one launch
+hipError_t launch_udt_qbuild(const UdtBatch& b, hipStream_t s);
 
 // rdivp: A <- A[:, pivot] * triu(T)^-1 with O as n*n scratch (qr.hip).
 struct RdivpItem {

@@ -391,9 +391,9 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
     Timed t(h, DQMC_KERNEL_UDT);
     for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
   }
-  for (int pi = np - 1; pi >= 0; pi--) {
+  {
     Timed t(h, DQMC_KERNEL_UDT_UPDATE);
-    HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], st[pi], 1, h->st));
+    HIPCHK(h, launch_udt_qbuild(u, h->st));
   }
   Timed t(h, DQMC_KERNEL_UDT);
   HIPCHK(h, launch_udt_finish(u, h->st));

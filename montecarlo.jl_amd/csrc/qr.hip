@@ -73,7 +73,9 @@ __device__ __forceinline__ double vget(const double* P, int rows, int w, int i, 
 // norms, it.pivot the working permutation, it.tfac the T factor of each panel.
 
 // Rows of the panel per thread of the panel kernel (256 threads): 1, 2, 4 or 8.
-static int udt_rows_per_thread(int rows) {
+constexpr int kMaxUdtPanels = 512;   // panels of an n <= 2048 UDT (widths >= 8 beyond 256 rows)
+
+__host__ __device__ static int udt_rows_per_thread(int rows) {
   int R = 1;
   while (R * 256 < rows) R *= 2;
   return R;
@@ -81,7 +83,7 @@ static int udt_rows_per_thread(int rows) {
 
 // Panel width for `rows` remaining rows: the panel lives in registers (R rows x w columns
 // per thread, at most 64 values) and in LDS (rows x w <= kPanelDoubles).
-static int udt_panel_width(int rows) {
+__host__ __device__ static int udt_panel_width(int rows) {
   int w = 64 / udt_rows_per_thread(rows);
   if (w > kQrMaxW) w = kQrMaxW;
   if (w > rows) w = rows;
@@ -568,15 +570,11 @@ __device__ __forceinline__ double vpanel(const double* Vp, int n, int rows, int 
   return Vp[(size_t)i + (size_t)a * n];
 }
 
-__global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int mode) {
-  const UdtItem it = batch.it[blockIdx.y];
-  const int n = batch.n;
+__device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, int w, int cb, int mode,
+                                              double (&red)[4][4][64], double (&nred)[4][16]) {
   const int rows = n - p0;
-  const int cb = c0 + 16 * blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  __shared__ double red[4][4][64];
-  __shared__ double nred[4][16];
   const bool trans = mode == 0;
   const double* Vp = it.A + (size_t)p0 * n + p0;
   double* __restrict__ X = mode == 0 ? it.A : it.U;
@@ -673,6 +671,41 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
   }
 }
 
+__global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int mode) {
+  __shared__ double red[4][4][64];
+  __shared__ double nred[4][16];
+  reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * blockIdx.x, mode, red, nred);
+}
+
+// Q = H_1 ... H_P by backward accumulation in one launch: the workgroup of U's columns
+// [16 s, 16 s + 16) applies the block reflectors of the panels from the last to the first.
+// Columns left of a panel's start are still unit columns there (zero in the reflector's
+// rows), so a strip is skipped only where it lies wholly left of the panel.
+__global__ __launch_bounds__(256) void udt_qbuild_kernel(UdtBatch batch) {
+  __shared__ double red[4][4][64];
+  __shared__ double nred[4][16];
+  __shared__ int pst[kMaxUdtPanels], pwd[kMaxUdtPanels];
+  __shared__ int np;
+  const int n = batch.n;
+  if (threadIdx.x == 0) {
+    int k = 0;
+    for (int p0 = 0; p0 < n && k < kMaxUdtPanels; k++) {
+      const int w = udt_panel_width(n - p0);
+      pst[k] = p0;
+      pwd[k] = w;
+      p0 += w;
+    }
+    np = k;
+  }
+  __syncthreads();
+  const int cb = 16 * blockIdx.x;
+  for (int pi = np - 1; pi >= 0; pi--) {
+    if (cb + 16 <= pst[pi]) continue;     // uniform across the workgroup
+    reflect_strip(batch.it[blockIdx.y], n, pst[pi], pwd[pi], cb, 1, red, nred);
+    __syncthreads();                      // red is reused by the next panel
+  }
+}
+
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
 // is copied), one wave per column.
 __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
@@ -752,6 +785,11 @@ hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode
   return hipGetLastError();
 }
 
+hipError_t launch_udt_qbuild(const UdtBatch& b, hipStream_t s) {
+  hipLaunchKernelGGL(udt_qbuild_kernel, dim3((b.n + 15) / 16, b.count), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s) {
   hipLaunchKernelGGL(udt_finish_kernel, dim3((b.n + 3) / 4, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
@@ -772,8 +810,7 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
   }
   for (int k = 0; k < b.count; k++)
     if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
-  for (int pi = np - 1; pi >= 0; pi--)
-    if ((e = launch_udt_reflect(b, st[pi], wd[pi], st[pi], 1, s)) != hipSuccess) return e;
+  if ((e = launch_udt_qbuild(b, s)) != hipSuccess) return e;
   return launch_udt_finish(b, s);
 }
 
