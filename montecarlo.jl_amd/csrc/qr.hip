@@ -121,21 +121,48 @@ __device__ __forceinline__ bool pivot_before(double a, int ia, double b, int ib)
   return a > b || (a == b && ia < ib);
 }
 
+// Value of lane (lane ^ S).  Inside a 16-lane row with DPP: xor 1 and xor 2 are quad
+// permutations, xor 4 a half-row mirror (i <-> 7 - i) followed by a quad reversal, xor 8
+// a row rotation by 8; across rows through ds_bpermute.
+template <int S>
+__device__ __forceinline__ int lane_xor(int x) {
+  if constexpr (S == 1) {
+    return __builtin_amdgcn_update_dpp(0, x, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+  } else if constexpr (S == 2) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+  } else if constexpr (S == 4) {
+    const int m = __builtin_amdgcn_update_dpp(0, x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    return __builtin_amdgcn_update_dpp(0, m, 0x1B, 0xF, 0xF, false);    // quad_perm [3,2,1,0]
+  } else if constexpr (S == 8) {
+    return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);   // row_ror 8
+  } else {
+    return __shfl_xor(x, S, 64);
+  }
+}
+
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ void bitonic_stage(double& kv, int& ki, int lane) {
+  const int lo = lane_xor<STRIDE>(__double2loint(kv));
+  const int hi = lane_xor<STRIDE>(__double2hiint(kv));
+  const double pv = __hiloint2double(hi, lo);
+  const int pi = lane_xor<STRIDE>(ki);
+  const bool up = (lane & SIZE) == 0 || SIZE == 64;
+  const bool lower = (lane & STRIDE) == 0;
+  const bool pb = pivot_before(pv, pi, kv, ki);
+  const bool take = lower == up ? pb : !pb;
+  kv = take ? pv : kv;
+  ki = take ? pi : ki;
+  if constexpr (STRIDE > 1) bitonic_stage<SIZE, STRIDE / 2>(kv, ki, lane);
+}
+
 // Bitonic sort of one (norm, column) pair per lane of a wave into pivot order.
 __device__ __forceinline__ void bitonic64(double& kv, int& ki, int lane) {
-#pragma unroll
-  for (int size = 2; size <= 64; size *= 2) {
-#pragma unroll
-    for (int stride = size / 2; stride > 0; stride /= 2) {
-      const double pv = __shfl_xor(kv, stride, 64);
-      const int pi = __shfl_xor(ki, stride, 64);
-      const bool up = (lane & size) == 0 || size == 64;
-      const bool lower = (lane & stride) == 0;
-      const bool pb = pivot_before(pv, pi, kv, ki);
-      const bool take = lower == up ? pb : !pb;
-      if (take) { kv = pv; ki = pi; }
-    }
-  }
+  bitonic_stage<2, 1>(kv, ki, lane);
+  bitonic_stage<4, 2>(kv, ki, lane);
+  bitonic_stage<8, 4>(kv, ki, lane);
+  bitonic_stage<16, 8>(kv, ki, lane);
+  bitonic_stage<32, 16>(kv, ki, lane);
+  bitonic_stage<64, 32>(kv, ki, lane);
 }
 
 static size_t udt_panel_lds_bytes(int n) {
@@ -152,72 +179,83 @@ static size_t udt_panel_lds_bytes(int n) {
 }
 
 // Column steps J.. of the single-wave panel factorisation (rows <= 256: lane l holds rows
-// l + 64 k).  A template recursion, so every register index is a compile-time constant.
+// l + 64 k; rows at or beyond `rows` hold zeros, so they add nothing and stay zero).  A
+// template recursion, so every register index is a compile-time constant, and branch-free
+// inside a step: only row block k = 0 holds rows <= j, selected per lane.
+// Per step: the partial sums of ||x||^2 and x^T a_c (c > j) are transposed through LDS
+// (lane 4 v + g adds value v of 16 lanes, quads combined with DPP) together with row j,
+// so lane 4 c holds column c's sum and a_jc and computes its w_c; the w_c reach every
+// lane by readlane.
 template <int J, int RT, int WM>
 __device__ __forceinline__ void panel_steps_wave(double (&a)[RT][WM], double (&taus)[WM], double* rw, int lane,
-                                                 int rows, int w) {
+                                                 int w) {
   if constexpr (J < WM) {
     constexpr int j = J;
     taus[j] = 0.0;
     if (j < w) {
+      double* rj = rw + 64 * 17;          // row j of the panel
       double part[WM];
+      {
+        const double x0 = a[0][j];
+        const double xe = lane >= j ? x0 : 0.0;
+        const double xg = lane > j ? x0 : 0.0;
+        part[j] = xe * xe;
 #pragma unroll
-      for (int c = 0; c < WM; c++) part[c] = 0.0;
-#pragma unroll
-      for (int k = 0; k < RT; k++) {
-        const int i = lane + 64 * k;
-        if (i < rows && i >= j) {
-          const double x = a[k][j];
-          part[j] += x * x;
-          if (i > j) {
-#pragma unroll
-            for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
-          }
-        }
+        for (int c = j + 1; c < WM; c++) part[c] = xg * a[0][c];
       }
 #pragma unroll
-      for (int c = 0; c < WM; c++) rw[lane * 17 + c] = part[c];
+      for (int k = 1; k < RT; k++) {
+        const double x = a[k][j];
+        part[j] += x * x;
+#pragma unroll
+        for (int c = j + 1; c < WM; c++) part[c] += x * a[k][c];
+      }
+#pragma unroll
+      for (int c = j; c < WM; c++) rw[lane * 17 + c] = part[c];
+      if (lane == j) {
+#pragma unroll
+        for (int c = j; c < WM; c++) rj[c] = a[0][c];
+      }
       __builtin_amdgcn_wave_barrier();
       const int v = lane >> 2, g = lane & 3;
       double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
       for (int q = 0; q < 16; q++) s4[q & 3] += rw[(g * 16 + q) * 17 + v];
+      const double ajv = rj[v];
       double sv = (s4[0] + s4[1]) + (s4[2] + s4[3]);
       __builtin_amdgcn_wave_barrier();
       sv += dpp_mov<0xB1>(sv);             // quad_perm [1,0,3,2]
       sv += dpp_mov<0x4E>(sv);             // quad_perm [2,3,0,1]
       const double normu = readlane_d(sv, 4 * j);
-      const double xi1 = readlane_d(a[0][j], j);   // row j < 64 lives in lane j, k = 0
-      double tau = 0.0, nu = 0.0, xi = 1.0;
-      if (normu != 0.0) {
-        nu = copysign(sqrt(normu), xi1);
-        xi = xi1 + nu;
-        tau = xi / nu;
-      }
-      const double rxi = 1.0 / xi;
-      double wc[WM];
+      const double xi1 = readlane_d(ajv, 4 * j);
+      if (normu != 0.0) {                  // wave-uniform
+        const double nu = copysign(sqrt(normu), xi1);
+        const double xi = xi1 + nu;
+        const double tau = xi / nu;
+        const double rxi = 1.0 / xi;
+        const double wl = tau * (ajv + sv * rxi);   // w_v in lanes 4 v .. 4 v + 3
+        double wc[WM];
 #pragma unroll
-      for (int c = 0; c < WM; c++)
-        wc[c] = c > j ? tau * (readlane_d(a[0][c], j) + readlane_d(sv, 4 * c) * rxi) : 0.0;
+        for (int c = j + 1; c < WM; c++) wc[c] = readlane_d(wl, 4 * c);
+        {
+          const double x0 = a[0][j];
+          const double vv = x0 * rxi;
+          const double coef = lane > j ? vv : (lane == j ? 1.0 : 0.0);
+          a[0][j] = lane > j ? vv : (lane == j ? -nu : x0);
 #pragma unroll
-      for (int k = 0; k < RT; k++) {
-        const int i = lane + 64 * k;
-        if (normu != 0.0 && i < rows) {
-          if (i > j) {
-            const double vv = a[k][j] * rxi;
-            a[k][j] = vv;
-#pragma unroll
-            for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
-          } else if (i == j) {
-            a[k][j] = -nu;
-#pragma unroll
-            for (int c = j + 1; c < WM; c++) a[k][c] -= wc[c];
-          }
+          for (int c = j + 1; c < WM; c++) a[0][c] -= coef * wc[c];
         }
+#pragma unroll
+        for (int k = 1; k < RT; k++) {
+          const double vv = a[k][j] * rxi;
+          a[k][j] = vv;
+#pragma unroll
+          for (int c = j + 1; c < WM; c++) a[k][c] -= vv * wc[c];
+        }
+        taus[j] = tau;
       }
-      taus[j] = tau;
     }
-    panel_steps_wave<J + 1, RT, WM>(a, taus, rw, lane, rows, w);
+    panel_steps_wave<J + 1, RT, WM>(a, taus, rw, lane, w);
   }
 }
 
@@ -424,18 +462,25 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   // otherwise all 256 threads, thread t holding rows t + 256 k (k < R)
   constexpr int RT = R == 1 ? 4 : R;         // rows per participating thread
   constexpr int TP = R == 1 ? 64 : 256;      // participating threads
+  // rows at or beyond `rows` are zero (the factorisation relies on it); threads beyond TP
+  // hold no rows
   double a[RT][WM];
-  {
-    const int tl = tid & (TP - 1);   // threads beyond TP load a copy and never use it
+  if (tid < TP) {
 #pragma unroll
     for (int k = 0; k < RT; k++) {
-      const int i = tl + TP * k;
+      const int i = tid + TP * k;
       const int ic = i < rows ? i : rows - 1;
 #pragma unroll
       for (int c = 0; c < WM; c++) {
         const int src = c < w ? selr[c] : selr[0];
         a[k][c] = A[(size_t)(p0 + ic) + (size_t)src * n];
       }
+    }
+#pragma unroll
+    for (int k = 0; k < RT; k++) {
+      const bool ok = tid + TP * k < rows;
+#pragma unroll
+      for (int c = 0; c < WM; c++) a[k][c] = ok ? a[k][c] : 0.0;
     }
   }
   int pnew = 0;
@@ -459,11 +504,11 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
 
   if constexpr (R == 1) {
-  // single-wave factorisation: per column one wave reduction (partial sums transposed
-  // through LDS, quads combined with DPP), row j read from its owner lane; no barrier
+  // single-wave factorisation: per column one wave reduction (partial sums and row j
+  // transposed through LDS, quads combined with DPP); no barrier
   if (wave == 0) {
     double taus[WM];
-    panel_steps_wave<0, RT, WM>(a, taus, red, lane, rows, w);
+    panel_steps_wave<0, RT, WM>(a, taus, red, lane, w);
     if (lane == 0) {
 #pragma unroll
       for (int j = 0; j < WM; j++) sc[j] = taus[j];
