@@ -381,10 +381,12 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   if (tid < kQrMaxW) selr[tid] = -1;
   __syncthreads();
 
-  // the w largest residual norms, ties to the lower index (reference: strict ">")
+  // the w largest residual norms, ties to the lower index (reference: strict ">"); wave 0
+  // lane t < w ends with the t-th choice in selk
+  int selk = p0;
   if (rows <= kQrThreads) {
     // bitonic sort of (norm, column) best-first inside each wave, the waves' top 16 merged
-    // by a second sort in wave 0
+    // by a second sort in wave 0, which keeps the result in registers
     const int k = p0 + tid;
     double kv = k < n ? nrm[k] : -HUGE_VAL;
     int ki = k < n ? k : n + tid;
@@ -398,31 +400,33 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       kv = candv[lane];
       ki = candi[lane];
       bitonic64(kv, ki, lane);
-      if (lane < w) selr[lane] = ki;
+      selk = ki;
     }
-  } else
-  for (int k = p0 + tid; k < n; k += kQrThreads) {
-    const double v = nrm[k];
-    int cnt = 0;
-    for (int k0 = p0; k0 < n; k0 += 16) {
-      double u[16];
+  } else {
+    for (int k = p0 + tid; k < n; k += kQrThreads) {
+      const double v = nrm[k];
+      int cnt = 0;
+      for (int k0 = p0; k0 < n; k0 += 16) {
+        double u[16];
 #pragma unroll
-      for (int q = 0; q < 16; q++) u[q] = nrm[k0 + q < n ? k0 + q : n - 1];
+        for (int q = 0; q < 16; q++) u[q] = nrm[k0 + q < n ? k0 + q : n - 1];
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int k2 = k0 + q;
-        cnt += k2 < n && ((u[q] > v) || (u[q] == v && k2 < k));
+        for (int q = 0; q < 16; q++) {
+          const int k2 = k0 + q;
+          cnt += k2 < n && ((u[q] > v) || (u[q] == v && k2 < k));
+        }
       }
+      if (cnt < w) selr[cnt] = k;
     }
-    if (cnt < w) selr[cnt] = k;
+    __syncthreads();
+    if (wave == 0) selk = lane < w ? selr[lane] : p0;
   }
-  __syncthreads();
 
   // Sequential swaps "slot p0+t <-> slot holding the t-th choice" simulated by wave 0 on
   // the at most 2w slots involved: lane q < w is slot p0+q, lane w+q the slot of choice q
   // when that lies outside the panel.  cont = original column now held by the lane's slot.
   if (wave == 0) {
-    const int selv = lane < w ? selr[lane] : p0;
+    const int selv = lane < w ? selk : p0;
     const bool ok = __ballot(lane < w && !(selv >= p0 && selv < n)) == 0ull;
     const int selc = ok ? selv : p0 + lane;                    // lane t: t-th choice
     // (all lanes take part in the shuffle; only then is the result selected)
@@ -483,6 +487,49 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       for (int c = 0; c < WM; c++) a[k][c] = ok ? a[k][c] : 0.0;
     }
   }
+  if constexpr (R == 1) {
+  // wave 0 factorises the panel while waves 1-3 move the columns.  Every source read of
+  // the panel registers is issued before the barrier and every column store after it
+  // (the vector memory operations of one workgroup reach memory in issue order).  The
+  // moves leave out the rows >= p0 of the panel's columns, which the panel store writes.
+  // Single-wave factorisation: per column one wave reduction (partial sums and row j
+  // transposed through LDS, quads combined with DPP); no barrier.
+  if (wave == 1) {
+    int pn = 0;
+    if (lane < nm) pn = perm[mv_src[lane]];
+    __builtin_amdgcn_wave_barrier();
+    if (lane < nm) perm[mv_dst[lane]] = pn;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double taus[WM];
+    panel_steps_wave<0, RT, WM>(a, taus, red, lane, w);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < WM; j++) sc[j] = taus[j];
+    }
+  } else {
+    const int t = tid - 64;
+    for (int r0 = 0; r0 < n; r0 += kQrThreads - 64) {
+      const int rr = r0 + t;
+      const int rc = rr < n ? rr : n - 1;
+      double v[2 * kQrMaxW];
+#pragma unroll
+      for (int q = 0; q < 2 * kQrMaxW; q++) {
+        const int src = q < nm ? mv_src[q] : p0;
+        v[q] = A[(size_t)rc + (size_t)src * n];
+      }
+#pragma unroll
+      for (int q = 0; q < 2 * kQrMaxW; q++) {
+        const int dst = mv_dst[q < nm ? q : 0];
+        const int lim = dst < p0 + w ? p0 : n;
+        if (q < nm && rr < lim) A[(size_t)rr + (size_t)dst * n] = v[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
+  } else {
   int pnew = 0;
   if (tid < nm) pnew = perm[mv_src[tid]];
   for (int r0 = 0; r0 < n; r0 += kQrThreads) {
@@ -502,20 +549,6 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   if (tid < nm) perm[mv_dst[tid]] = pnew;
   __syncthreads();
   for (int k = p0 + tid; k < n; k += kQrThreads) it.pivot[k] = perm[k];
-
-  if constexpr (R == 1) {
-  // single-wave factorisation: per column one wave reduction (partial sums and row j
-  // transposed through LDS, quads combined with DPP); no barrier
-  if (wave == 0) {
-    double taus[WM];
-    panel_steps_wave<0, RT, WM>(a, taus, red, lane, w);
-    if (lane == 0) {
-#pragma unroll
-      for (int j = 0; j < WM; j++) sc[j] = taus[j];
-    }
-  }
-  __syncthreads();
-  } else {
   // factorisation, one block reduction per column: partial sums go through LDS within
   // each wave (lane (v, g) adds value v of one group of lanes), the groups are combined
   // with DPP inside quads / half rows, the four wave totals through LDS and one barrier.
@@ -543,9 +576,9 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   // compact-WY factor T (upper triangular, 16 x 16, zero beyond w):
   // H_1..H_w = I - V T V^T.  S = V^T V on the matrix cores (the four waves take every
   // fourth k-step, partial sums reduced through LDS), then lane i of wave 0 builds row i
-  // of T from its own earlier entries, T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j],
-  // with S[k][j] taken from the accumulator registers by readlane (element k/4 of lane
-  // 16 (k%4) + j).
+  // of T from its own earlier entries, T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j]
+  // (T[i][k] = 0 for k < i, so the sum may start at 0), with S[k][j] read from LDS, the
+  // same address in every lane.
   __syncthreads();
   {
     dx4 Sacc = dx4{0, 0, 0, 0};
@@ -555,6 +588,7 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     }
     __syncthreads();   // every wave is done reading P: its space takes the partial sums
     double* Sred = P;  // [wave][r][lane]
+    double* Sm = P + 16 * 64;   // S, 16 x 16 row-major
     if (wave > 0) {
 #pragma unroll
       for (int r = 0; r < 4; r++) Sred[(wave * 4 + r) * 64 + lane] = Sacc[r];
@@ -562,21 +596,17 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
-      for (int r = 0; r < 4; r++)
+      for (int r = 0; r < 4; r++) {
         Sacc[r] += (Sred[(4 + r) * 64 + lane] + Sred[(8 + r) * 64 + lane]) + Sred[(12 + r) * 64 + lane];
-    }
-    if (wave == 0) {
+        Sm[(4 * r + (lane >> 4)) * 16 + (lane & 15)] = Sacc[r];   // element r: S[4r + l/16][l%16]
+      }
+      __builtin_amdgcn_wave_barrier();
       double row[16];
 #pragma unroll
       for (int j = 0; j < 16; j++) {
         double acc = 0.0;
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-          if (k < j) {
-            const double skj = readlane_d(Sacc[k >> 2], (k & 3) * 16 + j);
-            acc += (k >= lane ? row[k] : 0.0) * skj;
-          }
-        }
+        for (int k = 0; k < j; k++) acc += row[k] * Sm[k * 16 + j];
         const double tj = sc[j];
         double t = 0.0;
         if (lane < w && j < w) {
