@@ -387,7 +387,7 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
       HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], st[pi] + wd[pi], 0, h->st));
     }
   }
-  {
+  if (udt_qbuild_needs_identity(h->n)) {
     Timed t(h, DQMC_KERNEL_UDT);
     for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
   }

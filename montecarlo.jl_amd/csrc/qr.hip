@@ -664,7 +664,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
   }
   const int nblk = (rows + 15) / 16;
   const int ngroups = (nblk + 4 * kReflRB - 1) / (4 * kReflRB);
-  double xb[kReflRB][4], vb[kReflRB][4];
+  double xb[kReflRB][4], vb[kReflRB][4], va[kReflRB][4];
   dx4 Wacc = dx4{0, 0, 0, 0};
   for (int g = 0; g < ngroups; g++) {
 #pragma unroll
@@ -677,6 +677,14 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
         const double x = Xc[ic];
         xb[q][kc] = (i < rows && colok) ? x : 0.0;
         vb[q][kc] = vpanel(Vp, n, rows, w, i, li);
+      }
+    }
+    if (ngroups == 1) {      // the second product's operand, loaded with the first's
+#pragma unroll
+      for (int q = 0; q < kReflRB; q++) {
+        const int blk = wave + 4 * q;
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) va[q][kc] = vpanel(Vp, n, rows, w, 16 * blk + li, 4 * kc + lk);
       }
     }
 #pragma unroll
@@ -708,13 +716,12 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
           xb[q][kc] = (i < rows && colok) ? x : 0.0;
         }
       }
-    }
-    double va[kReflRB][4];
 #pragma unroll
-    for (int q = 0; q < kReflRB; q++) {
-      const int blk = g * 4 * kReflRB + wave + 4 * q;
+      for (int q = 0; q < kReflRB; q++) {
+        const int blk = g * 4 * kReflRB + wave + 4 * q;
 #pragma unroll
-      for (int kc = 0; kc < 4; kc++) va[q][kc] = vpanel(Vp, n, rows, w, 16 * blk + li, 4 * kc + lk);
+        for (int kc = 0; kc < 4; kc++) va[q][kc] = vpanel(Vp, n, rows, w, 16 * blk + li, 4 * kc + lk);
+      }
     }
 #pragma unroll
     for (int q = 0; q < kReflRB; q++) {
@@ -752,10 +759,70 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
   reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * blockIdx.x, mode, red, nred);
 }
 
+// Entry (i, a) of panel p0's reflector block V for absolute row i (unit diagonal, zero
+// above it and beyond the w reflectors), read with an unconditional in-bounds load (the
+// diagonal element when the entry is not stored) and then selected.
+__device__ __forceinline__ double vabs(const double* A, int n, int p0, int w, int i, int a) {
+  const int ac = a < w ? a : 0;
+  const bool stored = a < w && i > p0 + a && i < n;
+  const double x = A[(size_t)(stored ? i : p0 + ac) + (size_t)(p0 + ac) * n];
+  return stored ? x : (a < w && i == p0 + a ? 1.0 : 0.0);
+}
+
 // Q = H_1 ... H_P by backward accumulation in one launch: the workgroup of U's columns
 // [16 s, 16 s + 16) applies the block reflectors of the panels from the last to the first.
 // Columns left of a panel's start are still unit columns there (zero in the reflector's
 // rows), so a strip is skipped only where it lies wholly left of the panel.
+// n <= 256: the strip starts as the identity in registers (absolute rows, the f64 MFMA
+// B/D layout: wave v, group q holds the 16-row block v + 4 q) and stays there until the
+// end; each panel's V and T are loaded while the previous panel is applied.  Larger n:
+// one reflect_strip pass per panel over U, which then holds the identity beforehand.
+__device__ __forceinline__ void qbuild_load(const UdtItem& it, int n, int p0, int w, int wave, int li, int lk,
+                                            double (&vb)[4][4], double (&va)[4][4], double (&ta)[4]) {
+#pragma unroll
+  for (int kc = 0; kc < 4; kc++) ta[kc] = it.tfac[(size_t)p0 * 16 + li * 16 + 4 * kc + lk];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int blk = wave + 4 * q;
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+      vb[q][kc] = vabs(it.A, n, p0, w, 16 * blk + 4 * kc + lk, li);
+      va[q][kc] = vabs(it.A, n, p0, w, 16 * blk + li, 4 * kc + lk);
+    }
+  }
+}
+
+__device__ __forceinline__ void qbuild_apply(double (&xb)[4][4], const double (&vb)[4][4], const double (&va)[4][4],
+                                             const double (&ta)[4], double (&red)[4][4][64], int wave, int lane) {
+  dx4 W = dx4{0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) W = mfma16(vb[q][kc], xb[q][kc], W);
+#pragma unroll
+  for (int r = 0; r < 4; r++) red[wave][r][lane] = W[r];
+  __syncthreads();
+  dx4 Wt;
+#pragma unroll
+  for (int r = 0; r < 4; r++) Wt[r] = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
+  dx4 Y = dx4{0, 0, 0, 0};
+#pragma unroll
+  for (int kc = 0; kc < 4; kc++) Y = mfma16(ta[kc], Wt[kc], Y);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    dx4 acc;
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[r] = xb[q][r];
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) acc = mfma16(-va[q][kc], Y[kc], acc);
+#pragma unroll
+    for (int r = 0; r < 4; r++) xb[q][r] = acc[r];
+  }
+  __syncthreads();                      // red is rewritten by the next panel
+}
+
+__host__ __device__ inline bool udt_qbuild_in_registers(int n) { return n <= 256; }
+
 __global__ __launch_bounds__(256) void udt_qbuild_kernel(UdtBatch batch) {
   __shared__ double red[4][4][64];
   __shared__ double nred[4][16];
@@ -774,9 +841,43 @@ __global__ __launch_bounds__(256) void udt_qbuild_kernel(UdtBatch batch) {
   }
   __syncthreads();
   const int cb = 16 * blockIdx.x;
+  const UdtItem it = batch.it[blockIdx.y];
+  if (udt_qbuild_in_registers(n)) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int col = cb + li;
+    double xb[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int kc = 0; kc < 4; kc++) xb[q][kc] = 16 * (wave + 4 * q) + 4 * kc + lk == col ? 1.0 : 0.0;
+    int pmax = -1;                        // the panels applied: 0 .. pmax (starts below cb + 16)
+    for (int pi = 0; pi < np; pi++)
+      if (pst[pi] < cb + 16) pmax = pi;
+    double vb[4][4], va[4][4], ta[4], vb2[4][4], va2[4][4], ta2[4];
+    if (pmax >= 0) qbuild_load(it, n, pst[pmax], pwd[pmax], wave, li, lk, vb, va, ta);
+    // two panels per trip, so the prefetch buffers alternate without copies
+    for (int pi = pmax; pi >= 0; pi -= 2) {
+      if (pi >= 1) qbuild_load(it, n, pst[pi - 1], pwd[pi - 1], wave, li, lk, vb2, va2, ta2);
+      qbuild_apply(xb, vb, va, ta, red, wave, lane);
+      if (pi < 1) break;
+      if (pi >= 2) qbuild_load(it, n, pst[pi - 2], pwd[pi - 2], wave, li, lk, vb, va, ta);
+      qbuild_apply(xb, vb2, va2, ta2, red, wave, lane);
+    }
+    if (col < n) {
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int kc = 0; kc < 4; kc++) {
+          const int i = 16 * (wave + 4 * q) + 4 * kc + lk;
+          if (i < n) it.U[(size_t)i + (size_t)col * n] = xb[q][kc];
+        }
+    }
+    return;
+  }
   for (int pi = np - 1; pi >= 0; pi--) {
     if (cb + 16 <= pst[pi]) continue;     // uniform across the workgroup
-    reflect_strip(batch.it[blockIdx.y], n, pst[pi], pwd[pi], cb, 1, red, nred);
+    reflect_strip(it, n, pst[pi], pwd[pi], cb, 1, red, nred);
     __syncthreads();                      // red is reused by the next panel
   }
 }
@@ -860,6 +961,8 @@ hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode
   return hipGetLastError();
 }
 
+bool udt_qbuild_needs_identity(int n) { return !udt_qbuild_in_registers(n); }
+
 hipError_t launch_udt_qbuild(const UdtBatch& b, hipStream_t s) {
   hipLaunchKernelGGL(udt_qbuild_kernel, dim3((b.n + 15) / 16, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
@@ -883,8 +986,9 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
     if ((e = launch_udt_panel(b, st[pi], wd[pi], pi == np - 1, s)) != hipSuccess) return e;
     if ((e = launch_udt_reflect(b, st[pi], wd[pi], st[pi] + wd[pi], 0, s)) != hipSuccess) return e;
   }
-  for (int k = 0; k < b.count; k++)
-    if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
+  if (!udt_qbuild_in_registers(b.n))
+    for (int k = 0; k < b.count; k++)
+      if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
   if ((e = launch_udt_qbuild(b, s)) != hipSuccess) return e;
   return launch_udt_finish(b, s);
 }
