@@ -1110,31 +1110,51 @@ __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
   }
   __syncthreads();
 
+  // T of the next panel (its diagonal block and coupling rows k < 256) is loaded while the
+  // current panel is solved
+  constexpr int kPre = 64;                       // prefetched coupling rows k = 4 q + lk
+  double td[4], tb[kPre];
+  auto load_t = [&](int j0n) {
+    const int cbn = j0n + li;
+    const int cbcn = cbn < n ? cbn : n - 1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int a = lk + 4 * q;
+      const int ta = j0n + a < n ? j0n + a : n - 1;
+      const double x = T[(size_t)ta + (size_t)cbcn * n];
+      td[q] = (j0n + a < n && cbn < n) ? x : (a == li ? 1.0 : 0.0);   // out of range: identity
+    }
+#pragma unroll
+    for (int g = 0; g < kPre / 16; g++) {
+      if (64 * g < j0n) {                        // uniform: 16 loads at once
+#pragma unroll
+        for (int q = 16 * g; q < 16 * g + 16; q++) {
+          const int k = 4 * q + lk;
+          tb[q] = T[(size_t)(k < j0n ? k : 0) + (size_t)cbcn * n];
+        }
+      }
+    }
+  };
+  load_t(0);
   for (int j0 = 0; j0 < n; j0 += 16) {
     const int w = n - j0 < 16 ? n - j0 : 16;
     const int cb = j0 + li;
     const int cbc = cb < n ? cb : n - 1;
-    // diagonal block of T (out of range: identity)
-    double td[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int a = lk + 4 * q;
-      const int ta = j0 + a < n ? j0 + a : n - 1;
-      const double x = T[(size_t)ta + (size_t)cbc * n];
-      td[q] = (j0 + a < n && cb < n) ? x : (a == li ? 1.0 : 0.0);
-    }
     dx4 acc = dx4{0, 0, 0, 0};
-    for (int kb = 0; kb < j0; kb += 128) {
-      double tb[32];
+#pragma unroll
+    for (int q = 0; q < kPre; q++)
+      if (4 * q < j0) acc = mfma16(X[(size_t)(4 * q + lk) * 17 + li], tb[q], acc);
+    for (int kb = 4 * kPre; kb < j0; kb += 128) {
+      double tc[32];
 #pragma unroll
       for (int q = 0; q < 32; q++) {
         const int k = kb + 4 * q + lk;
-        tb[q] = T[(size_t)(k < j0 ? k : 0) + (size_t)cbc * n];
+        tc[q] = T[(size_t)(k < j0 ? k : 0) + (size_t)cbc * n];
       }
 #pragma unroll
       for (int q = 0; q < 32; q++) {
         const int k = kb + 4 * q + lk;
-        if (kb + 4 * q < j0) acc = mfma16(X[(size_t)k * 17 + li], tb[q], acc);
+        if (kb + 4 * q < j0) acc = mfma16(X[(size_t)k * 17 + li], tc[q], acc);
       }
     }
 #pragma unroll
@@ -1144,16 +1164,32 @@ __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
       const int row = mfma_row<double>(lane, r);
       Ysh[row * 17 + li] = cb < n ? X[(size_t)cb * 17 + row] - acc[r] : 0.0;
     }
+    if (j0 + 16 < n) load_t(j0 + 16);
     __syncthreads();
     if (lane < 16) {
-      double x[16];
+      // right-looking: x_c = v_c / T_cc, then v_c' -= x_c T_cc' for every later c'.  The
+      // quotient comes from the reciprocal y = 1 / T_cc with one correction,
+      // x = q + (v - T_cc q) y, q = v y, which is the correctly rounded v / T_cc (Markstein)
+      // for the unit-modulus diagonal of a UDT's T; any other diagonal value divides.
+      const double dl = Tsh[lane * 17 + lane];
+      const double yl = 1.0 / dl;
+      double v[16], x[16];
+#pragma unroll
+      for (int c = 0; c < 16; c++) v[c] = Ysh[lane * 17 + c];
 #pragma unroll
       for (int c = 0; c < 16; c++) {
-        double v = Ysh[lane * 17 + c];
+        const double d = readlane_d(dl, c);
+        const double y = readlane_d(yl, c);
+        double xc;
+        if (fabs(d) >= 0.5 && fabs(d) <= 2.0) {
+          const double q0 = v[c] * y;
+          xc = fma(fma(-d, q0, v[c]), y, q0);
+        } else {
+          xc = v[c] / d;
+        }
+        x[c] = xc;
 #pragma unroll
-        for (int cp = 0; cp < 16; cp++)
-          if (cp < c) v -= x[cp] * Tsh[cp * 17 + c];
-        x[c] = v / Tsh[c * 17 + c];
+        for (int c2 = c + 1; c2 < 16; c2++) v[c2] -= xc * Tsh[c * 17 + c2];
       }
 #pragma unroll
       for (int c = 0; c < 16; c++)

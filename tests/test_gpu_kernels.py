@@ -84,11 +84,14 @@ def test_udt_dirty_upper(hip, n):
     assert np.abs(recon - X).max() < 1e-13 * np.abs(X).max() * max(1, n / 16)
 
 
-@pytest.mark.parametrize("n", [1, 16, 36, 256])
-def test_rdivp(hip, oracle, n):
+@pytest.mark.parametrize("n,unit", [(1, True), (16, True), (36, True), (256, True), (300, True),
+                                    (576, True), (16, False), (256, False)])
+def test_rdivp(hip, oracle, n, unit):
     rng = np.random.default_rng(n)
     A = np.asfortranarray(rng.standard_normal((n, n)))
-    T = np.asfortranarray(np.triu(rng.uniform(-0.5, 0.5, (n, n)), 1) + np.diag(rng.choice([-1.0, 1.0], n))
+    # a UDT's T has a unit-modulus diagonal; other diagonals exercise the plain division
+    diag = rng.choice([-1.0, 1.0], n) * (1.0 if unit else rng.choice([0.3, 0.7, 1.5, 3.0], n))
+    T = np.asfortranarray(np.triu(rng.uniform(-0.5, 0.5, (n, n)), 1) + np.diag(diag)
                           + np.tril(rng.standard_normal((n, n)), -1))  # dirty lower part
     piv = rng.permutation(n).astype(np.int32)
     out = hip.la_rdivp(A, T, piv)
