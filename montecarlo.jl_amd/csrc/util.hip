@@ -124,30 +124,39 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Single-workgroup reductions: results are deterministic (fixed reduction tree).
-__global__ __launch_bounds__(1024) void maxabsdiff_kernel(const double* __restrict__ a,
-                                                          const float* __restrict__ a32,
-                                                          const double* __restrict__ b, int64_t count,
-                                                          double* out) {
-  __shared__ double part[16];
+// max(x, y) with a NaN in either argument winning
+__device__ inline double nan_max(double x, double y) { return (y > x || y != y) ? y : x; }
+
+// max |a - b| over count elements (a from a32 when given): grid-stride partial maxima, one
+// per workgroup, combined with a 64-bit atomic max on the bit patterns, which order
+// non-negative doubles as their values (a NaN, positive after fabs, above +inf).  The
+// maximum is exact, so the result does not depend on the order.  *out must hold +0.0.
+__global__ __launch_bounds__(256) void maxabsdiff_kernel(const double* __restrict__ a,
+                                                         const float* __restrict__ a32,
+                                                         const double* __restrict__ b, int64_t count,
+                                                         double* out) {
+  __shared__ double part[4];
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < count; i += blockDim.x) {
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256) {
     const double av = a32 ? (double)a32[i] : a[i];
-    const double d = fabs(av - b[i]);
-    m = (d > m || d != d) ? d : m;
+    m = nan_max(m, fabs(av - b[i]));
   }
-  m = wave_max(m);
+  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) r = (part[w] > r || part[w] != part[w]) ? part[w] : r;
-    *out = r;
+    const double r = nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3]));
+    atomicMax(reinterpret_cast<unsigned long long*>(out), (unsigned long long)__double_as_longlong(r));
   }
 }
 hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b, int64_t count,
                              double* out, hipStream_t s) {
-  hipLaunchKernelGGL(maxabsdiff_kernel, dim3(1), dim3(1024), 0, s, a, a32, b, count, out);
+  hipError_t e = launch_fill(out, 0.0, 1, s);
+  if (e != hipSuccess) return e;
+  int64_t g = (count + 255) / 256;
+  if (g > 128) g = 128;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(maxabsdiff_kernel, dim3((unsigned)g), dim3(256), 0, s, a, a32, b, count, out);
   return hipGetLastError();
 }
 

@@ -185,6 +185,10 @@ def test_config_b_sweep_parity(pkg, hip, oracle):
     cg, co = g.counters(), o.counters()
     print("config B: mismatches", mism, "first", first, "ties", cg["ties"], co["ties"],
           "acc", cg["accepted"], co["accepted"], "prop_err", cg["prop_err_last"], co["prop_err_last"])
+    # the device's wrap-vs-recompute check (max |G_wrapped - G_fresh| at the last
+    # stabilisation) measures the same drift as the oracle's, to within rounding order
+    assert 0.0 < cg["prop_err_last"] < 1e-6 and 0.0 < co["prop_err_last"] < 1e-6
+    assert 0.1 < cg["prop_err_last"] / co["prop_err_last"] < 10.0
     if cg["ties"] == 0 and co["ties"] == 0:
         assert mism == 0
         assert np.array_equal(g.get_field(), o.get_field())
