@@ -88,6 +88,14 @@ int dqmc_get_hopping_exp(dqmc_chain_t h, double* exp_mhalf, double* exp_phalf, d
 /* field.conf (fields.jl:329-334), N x M Int8 +-1. */
 int dqmc_set_field(dqmc_chain_t h, const int8_t* conf, int ld);
 int dqmc_get_field(dqmc_chain_t h, int8_t* conf, int ld);
+/* The field in the reference's compressed form: compress(f) = BitArray(f.conf .== 1),
+ * decompress!(f, c) = (f.conf .= Int8.(2c .- 1)) (fields.jl:331-334), as Julia stores a
+ * BitArray: nchunks = ceil(n_sites * slices / 64) uint64 chunks, bit k % 64 of chunk k / 64
+ * for column-major entry k, least significant bit first, high bits of the last chunk zero
+ * (set_field_packed refuses others).  Packed and unpacked on the device; set, then
+ * build_stack, resumes a chain from a stored field.                                      */
+int dqmc_get_field_packed(dqmc_chain_t h, uint64_t* chunks, int64_t nchunks);
+int dqmc_set_field_packed(dqmc_chain_t h, const uint64_t* chunks, int64_t nchunks);
 
 /* Randomness of the Metropolis test `p > 1 || rand() < p` (local_updates.jl:53): a
  * uniform is consumed only when p <= 1.  Either a supplied stream (copied to the

@@ -49,7 +49,8 @@ class Stats(C.Structure):
 EXPORTED = [
     "dqmc_create", "dqmc_destroy", "dqmc_last_error", "dqmc_footprint_bytes",
     "dqmc_set_hopping", "dqmc_set_hopping_exp", "dqmc_get_hopping_exp",
-    "dqmc_set_field", "dqmc_get_field", "dqmc_set_uniforms", "dqmc_set_seed",
+    "dqmc_set_field", "dqmc_get_field", "dqmc_get_field_packed", "dqmc_set_field_packed",
+    "dqmc_set_uniforms", "dqmc_set_seed",
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
@@ -65,6 +66,7 @@ KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local", "local_apply", 
 _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
 _i8 = np.ctypeslib.ndpointer(dtype=np.int8, flags="F_CONTIGUOUS")
+_u64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
@@ -87,6 +89,8 @@ def load():
         "dqmc_get_hopping_exp": [vp, _dp, _dp, _dp, _dp, i],
         "dqmc_set_field": [vp, _i8, i],
         "dqmc_get_field": [vp, _i8, i],
+        "dqmc_get_field_packed": [vp, _u64, i64],
+        "dqmc_set_field_packed": [vp, _u64, i64],
         "dqmc_set_uniforms": [vp, C.POINTER(d), i64],
         "dqmc_set_seed": [vp, C.c_uint64, C.c_uint64],
         "dqmc_set_decision_log": [vp, i64],
@@ -289,6 +293,17 @@ class HIPChain:
         c = np.zeros((self.n, self.M), dtype=np.int8, order="F")
         self._c(load().dqmc_get_field(self._h, c, self.n), "dqmc_get_field")
         return c
+
+    def get_field_packed(self):
+        """The field as BitArray chunks, packed on the device (fields.jl:331-334)."""
+        c = np.zeros((self.n * self.M + 63) // 64, dtype=np.uint64)
+        self._c(load().dqmc_get_field_packed(self._h, c, c.size), "dqmc_get_field_packed")
+        return c
+
+    def set_field_packed(self, chunks):
+        """Set the field from BitArray chunks, unpacked on the device (decompress!)."""
+        c = np.ascontiguousarray(chunks, dtype=np.uint64)
+        self._c(load().dqmc_set_field_packed(self._h, c, c.size), "dqmc_set_field_packed")
 
     def set_uniforms(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

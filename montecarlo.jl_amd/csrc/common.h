@@ -118,6 +118,9 @@ hipError_t launch_set_identity(double* A, int n, int nmat, hipStream_t s);
 hipError_t launch_set_identity_f32(float* A, int n, int nmat, hipStream_t s);
 hipError_t launch_fill(double* x, double v, int64_t count, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
+// field <-> BitArray chunks (64 entries per uint64, least significant bit first)
+hipError_t launch_pack_field(const int8_t* conf, int64_t count, uint64_t* chunks, hipStream_t s);
+hipError_t launch_unpack_field(const uint64_t* chunks, int64_t count, int8_t* conf, hipStream_t s);
 hipError_t launch_f64_to_f32(float* dst, const double* src, int64_t count, hipStream_t s);
 hipError_t launch_f32_to_f64(double* dst, const float* src, int64_t count, hipStream_t s);
 hipError_t launch_symmetrize(double* A, int n, hipStream_t s);

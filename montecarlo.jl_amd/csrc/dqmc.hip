@@ -60,6 +60,7 @@ struct dqmc_chain_s {
   int* piv = nullptr;
   double *tau = nullptr, *tfac = nullptr, *swp = nullptr;
   int8_t* conf = nullptr;
+  uint64_t* cpack = nullptr;   // the field as BitArray chunks (ceil(n M / 64))
   float *G32 = nullptr, *Gx32 = nullptr, *Gchk32 = nullptr, *eT2m32 = nullptr, *eT2p32 = nullptr;
   double* Gpp = nullptr;   // second copy of G for the window updates (FP64 form)
   float* G32pp = nullptr;  // second copy of G32 (mixed form)
@@ -758,6 +759,7 @@ int64_t footprint(const dqmc_config_t* c, int* Kout) {
   b += 8 * f * nn * 8 + 2 * f * n * 8;               // Ul Ur Tl Tr tmp1 tmp2 cur O, Dl Dr
   b += f * n * (4 + 8 + 32 * 8) + f * (n + 16) * 16 * 8; // pivot, tau, swap, tfac
   b += n * (int64_t)c->slices;                       // conf
+  b += (n * (int64_t)c->slices + 63) / 64 * 8;       // conf as BitArray chunks
   if (c->precision == DQMC_PRECISION_MIXED) b += (3 * f * nn + 2 * nn) * 4;
   const int64_t W = window_size((int)n, (int)f);
   const int64_t welem = c->precision == DQMC_PRECISION_MIXED ? 4 : 8;
@@ -835,6 +837,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   A(&h->tmp1, f * nn); A(&h->tmp2, f * nn); A(&h->cur, f * nn); A(&h->O, f * nn);
   A(&h->piv, f * n); A(&h->tau, f * n); A(&h->tfac, f * (n + 16) * 16); A(&h->swp, f * 32 * n);
   A(&h->conf, n * (size_t)h->M);
+  A(&h->cpack, (n * (size_t)h->M + 63) / 64);
   if (h->prec == DQMC_PRECISION_MIXED) {
     A(&h->G32, f * nn); A(&h->Gx32, f * nn); A(&h->Gchk32, f * nn); A(&h->G32pp, f * nn);
     A(&h->eT2m32, nn); A(&h->eT2p32, nn);
@@ -970,6 +973,37 @@ int dqmc_get_field(dqmc_chain_t h, int8_t* conf, int ld) {
   ENTER(h);
   if (!conf || ld < h->n) { h->err = "dqmc_get_field: null conf or ld < n"; return DQMC_E_INVALID; }
   HIPCHK(h, hipMemcpy2DAsync(conf, ld, h->conf, h->n, h->n, h->M, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return DQMC_OK;
+}
+
+int dqmc_get_field_packed(dqmc_chain_t h, uint64_t* chunks, int64_t nchunks) {
+  ENTER(h);
+  const int64_t count = (int64_t)h->n * h->M, need = (count + 63) / 64;
+  if (!chunks || nchunks != need) {
+    h->err = "dqmc_get_field_packed: null chunks or nchunks != ceil(n_sites * slices / 64)";
+    return DQMC_E_INVALID;
+  }
+  HIPCHK(h, launch_pack_field(h->conf, count, h->cpack, h->st));
+  HIPCHK(h, hipMemcpyAsync(chunks, h->cpack, sizeof(uint64_t) * need, hipMemcpyDeviceToHost, h->st));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return DQMC_OK;
+}
+
+int dqmc_set_field_packed(dqmc_chain_t h, const uint64_t* chunks, int64_t nchunks) {
+  ENTER(h);
+  const int64_t count = (int64_t)h->n * h->M, need = (count + 63) / 64;
+  if (!chunks || nchunks != need) {
+    h->err = "dqmc_set_field_packed: null chunks or nchunks != ceil(n_sites * slices / 64)";
+    return DQMC_E_INVALID;
+  }
+  const int tail = (int)(count % 64);
+  if (tail && (chunks[need - 1] >> tail) != 0) {
+    h->err = "dqmc_set_field_packed: bits set beyond the field in the last chunk";
+    return DQMC_E_INVALID;
+  }
+  HIPCHK(h, hipMemcpyAsync(h->cpack, chunks, sizeof(uint64_t) * need, hipMemcpyHostToDevice, h->st));
+  HIPCHK(h, launch_unpack_field(h->cpack, count, h->conf, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return DQMC_OK;
 }

@@ -42,6 +42,40 @@ hipError_t launch_fill(double* x, double v, int64_t count, hipStream_t s) {
   return hipGetLastError();
 }
 
+// The Hirsch field in the reference's compressed form, BitArray(conf .== 1)
+// (fields.jl:331-334): bit k % 64 of chunk k / 64 is set where the column-major entry k is
+// +1; the high bits of the last chunk beyond the field are zero.  Pack: one wave per chunk,
+// lane l reads entry 64 c + l and the wave's ballot is the chunk.
+__global__ __launch_bounds__(256) void pack_field_kernel(const int8_t* __restrict__ conf, int64_t count,
+                                                         uint64_t* __restrict__ chunks, int64_t nchunks) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * 4;
+  for (int64_t c = blockIdx.x * (int64_t)4 + (threadIdx.x >> 6); c < nchunks; c += waves) {
+    const int64_t k = c * 64 + lane;
+    const int8_t x = conf[k < count ? k : count - 1];
+    const unsigned long long bits = __ballot(k < count && x == 1);   // whole wave active
+    if (lane == 0) chunks[c] = (uint64_t)bits;
+  }
+}
+__global__ void unpack_field_kernel(const uint64_t* __restrict__ chunks, int64_t count, int8_t* __restrict__ conf) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < count;
+       k += (int64_t)gridDim.x * blockDim.x)
+    conf[k] = ((chunks[k >> 6] >> (k & 63)) & 1ull) ? int8_t(1) : int8_t(-1);
+}
+hipError_t launch_pack_field(const int8_t* conf, int64_t count, uint64_t* chunks, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  const int64_t nchunks = (count + 63) / 64;
+  int64_t g = (nchunks + 3) / 4;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(pack_field_kernel, dim3((unsigned)g), dim3(256), 0, s, conf, count, chunks, nchunks);
+  return hipGetLastError();
+}
+hipError_t launch_unpack_field(const uint64_t* chunks, int64_t count, int8_t* conf, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(unpack_field_kernel, grid_for(count), dim3(256), 0, s, chunks, count, conf);
+  return hipGetLastError();
+}
+
 __global__ void copy_kernel(double* __restrict__ dst, const double* __restrict__ src, int64_t count) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
        i += (int64_t)gridDim.x * blockDim.x)

@@ -44,6 +44,28 @@ def field_from_seed(seed: int, N: int, M: int) -> np.ndarray:
     return np.asfortranarray(conf.reshape((M, N)).T)
 
 
+def bits_to_chunks(bits: np.ndarray) -> np.ndarray:
+    """A Bool array in Julia's BitArray storage: the column-major entries in uint64 chunks,
+    entry k at bit k % 64 of chunk k // 64, the unused high bits of the last chunk zero."""
+    flat = np.asarray(bits, dtype=bool).ravel(order="F")
+    nch = (flat.size + 63) // 64
+    padded = np.zeros(nch * 64, dtype=np.uint64)
+    padded[:flat.size] = flat
+    weights = np.left_shift(np.uint64(1), np.arange(64, dtype=np.uint64))
+    return (padded.reshape(nch, 64) * weights).sum(axis=1, dtype=np.uint64)
+
+
+def chunks_to_bits(chunks: np.ndarray, shape) -> np.ndarray:
+    """Inverse of bits_to_chunks for an array of the given shape (column-major)."""
+    chunks = np.asarray(chunks, dtype=np.uint64)
+    count = int(np.prod(shape))
+    if chunks.size != (count + 63) // 64:
+        raise ValueError("chunk count does not match the shape")
+    k = np.arange(count, dtype=np.uint64)
+    bits = (chunks[k >> np.uint64(6)] >> (k & np.uint64(63))) & np.uint64(1)
+    return bits.astype(bool).reshape(shape, order="F")
+
+
 def _real_alpha(x: complex, name: str) -> float:
     # maybe_to_float, fields.jl:294
     if abs(x.imag) < 10 * np.finfo(float).eps * max(abs(x.real), np.finfo(float).tiny):
@@ -68,6 +90,14 @@ class AbstractHirschField:
 
     def decompress(self, c: np.ndarray) -> None:
         self.conf[:, :] = (2 * np.asarray(c, dtype=np.int8) - 1)
+
+    def compress_chunks(self) -> np.ndarray:
+        """compress(f) as Julia stores the BitArray: uint64 chunks (fields.jl:331)."""
+        return bits_to_chunks(self.compress())
+
+    def decompress_chunks(self, chunks: np.ndarray) -> None:
+        """decompress!(f, c) from BitArray chunks (fields.jl:334)."""
+        self.decompress(chunks_to_bits(chunks, self.conf.shape))
 
     def energy_boson(self) -> float:
         return 0.0

@@ -283,3 +283,42 @@ def test_driver_device_measurements_match_host_path(pkg):
     for k in ("density", "kinetic_energy", "interaction_energy", "total_energy"):
         assert abs(ma.mean(k) - mb.mean(k)) <= 1e-11 * max(1.0, abs(ma.mean(k)))
     assert np.allclose(ma.mean("double_occupancy"), mb.mean("double_occupancy"), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("L,beta", [(6, 1.5), (16, 10.0), (3, 0.7)])
+def test_field_packed_on_device(pkg, hip, L, beta):
+    """The field as BitArray chunks (compress / decompress!, fields.jl:331-334), packed and
+    unpacked on the device, exact and ragged sizes; a chain resumed from the chunks builds
+    the same G."""
+    g, model, par, field = make_chain(hip.HIPChain, pkg, L, -4.0, beta, seed=41)
+    h = None
+    try:
+        g.set_hopping(model.hopping_matrix())
+        g.set_field(field.conf)
+        assert np.array_equal(g.get_field_packed(), field.compress_chunks())
+        g.set_seed(41)
+        g.build_stack()
+        g.sweep(1)
+        chunks = g.get_field_packed()
+        assert np.array_equal(chunks, pkg.fields.bits_to_chunks(g.get_field() == 1))
+        # resume in a fresh chain from the chunks
+        h, _, _, _ = make_chain(hip.HIPChain, pkg, L, -4.0, beta, seed=41)
+        h.set_hopping(model.hopping_matrix())
+        h.set_field_packed(chunks)
+        assert np.array_equal(h.get_field(), g.get_field())
+        g.build_stack()
+        h.build_stack()
+        assert np.array_equal(h.greens(), g.greens())
+        # refusals: wrong chunk count, bits beyond the field in the last chunk
+        with pytest.raises(hip.DQMCError):
+            h.set_field_packed(chunks[:-1])
+        count = g.n * g.M
+        if count % 64:
+            bad = chunks.copy()
+            bad[-1] |= np.uint64(1) << np.uint64(63)
+            with pytest.raises(hip.DQMCError):
+                h.set_field_packed(bad)
+    finally:
+        g.close()
+        if h is not None:
+            h.close()

@@ -91,6 +91,36 @@ def test_hirsch_fields(pkg):
     assert np.array_equal(fa.conf, fa2.conf)
 
 
+def test_field_chunks_match_julia_bitarray_layout(pkg):
+    """compress(f) = BitArray(conf .== 1) (fields.jl:331) in Julia's storage: column-major
+    entry k at bit k % 64 of uint64 chunk k // 64, unused high bits zero."""
+    from montecarlo_jl_amd.fields import bits_to_chunks, chunks_to_bits
+    conf = np.array([[1, -1], [-1, -1], [1, 1]], dtype=np.int8, order="F")   # entries 0, 2, 5 are +1
+    assert bits_to_chunks(conf == 1).tolist() == [1 + 4 + 32]
+    b = np.zeros(65, dtype=bool)
+    b[63] = b[64] = True
+    assert bits_to_chunks(b).tolist() == [1 << 63, 1]
+    # against numpy's little-endian bit packing (a library routine), ragged lengths
+    rng = np.random.default_rng(3)
+    for shape in [(16, 10), (36, 15), (7, 9), (256, 100), (1, 1)]:
+        bits = rng.random(shape) < 0.5
+        flat = bits.ravel(order="F")
+        nch = (flat.size + 63) // 64
+        by = np.zeros(nch * 8, dtype=np.uint8)
+        pk = np.packbits(flat, bitorder="little")
+        by[:pk.size] = pk
+        assert np.array_equal(bits_to_chunks(bits), by.view("<u8"))
+        assert np.array_equal(chunks_to_bits(bits_to_chunks(bits), shape), bits)
+    par = pkg.DQMCParameters.create(beta=1.5, delta_tau=0.1)
+    model = pkg.AttractiveHubbardModel(6, 2, U=4.0)
+    f = pkg.DensityHirschField(par, model)
+    f.rand(9)
+    c = f.compress_chunks()
+    g = pkg.DensityHirschField(par, model)
+    g.decompress_chunks(c)
+    assert c.size == (36 * 15 + 63) // 64 and np.array_equal(f.conf, g.conf)
+
+
 def test_uniform_stream_matches_oracle(oracle):
     from montecarlo_jl_amd.fields import uniforms
     u = uniforms(1234, 10, 50)
