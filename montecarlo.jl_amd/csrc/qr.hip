@@ -58,18 +58,11 @@ __device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// Entry (i, a) of the unit lower-trapezoidal reflector block V (panel-local row i),
-// zero outside the w reflectors of the panel.
-__device__ __forceinline__ double vget(const double* P, int rows, int w, int i, int a) {
-  if (i >= rows || a >= w) return 0.0;
-  return i == a ? 1.0 : (i > a ? P[i + (size_t)rows * a] : 0.0);
-}
-
 // The UDT runs as a sequence of launches per batch (one stream, so each sees the previous
 // one's results): column norms; per panel a single-workgroup panel kernel (pivot choice,
 // column moves, Householder factorisation of the panel in LDS, compact-WY factor T) and a
-// many-workgroup trailing update; Q by backward accumulation, one many-workgroup launch
-// per panel; finally D and T.  Per-item state between launches: it.tau holds the residual
+// many-workgroup trailing update; Q by backward accumulation in one many-workgroup launch;
+// finally D and T.  Per-item state between launches: it.tau holds the residual
 // norms, it.pivot the working permutation, it.tfac the T factor of each panel.
 
 // Rows of the panel per thread of the panel kernel (256 threads): 1, 2, 4 or 8.
@@ -582,9 +575,25 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   __syncthreads();
   {
     dx4 Sacc = dx4{0, 0, 0, 0};
-    for (int k0 = 4 * wave; k0 < rows; k0 += 16) {
-      const double v = vget(P, rows, w, k0 + (lane >> 4), lane & 15);
-      Sacc = mfma16(v, v, Sacc);
+    // 16 k-steps of this wave at a time: every LDS read issued (clamped addresses) before
+    // the entries of V are selected and the products run, in k order
+    for (int base = 4 * wave; base < rows; base += 256) {
+      const int a = lane & 15;
+      const int ac = a < w ? a : 0;
+      double vv[16];
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = base + 16 * q + (lane >> 4);
+        vv[q] = P[(i < rows ? i : rows - 1) + (size_t)rows * ac];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        const int i = base + 16 * q + (lane >> 4);
+        vv[q] = (i < rows && a < w) ? (i == a ? 1.0 : (i > a ? vv[q] : 0.0)) : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; q++)
+        if (base + 16 * q < rows) Sacc = mfma16(vv[q], vv[q], Sacc);
     }
     __syncthreads();   // every wave is done reading P: its space takes the partial sums
     double* Sred = P;  // [wave][r][lane]
