@@ -378,21 +378,26 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   // lane t < w ends with the t-th choice in selk
   int selk = p0;
   if (rows <= kQrThreads) {
-    // bitonic sort of (norm, column) best-first inside each wave, the waves' top 16 merged
-    // by a second sort in wave 0, which keeps the result in registers
+    // bitonic sort of (norm, column) best-first inside each wave; the waves' top 16 are
+    // merged in wave 0, which keeps the result in registers.  With the odd waves' lists
+    // stored reversed, the 64 candidates are in the state the bitonic network reaches after
+    // its size-16 stage (runs alternately best-first and reversed), so only the size-32 and
+    // size-64 merges remain.
     const int k = p0 + tid;
     double kv = k < n ? nrm[k] : -HUGE_VAL;
     int ki = k < n ? k : n + tid;
     bitonic64(kv, ki, lane);
     if (lane < 16) {
-      candv[wave * 16 + lane] = kv;
-      candi[wave * 16 + lane] = ki;
+      const int slot = wave * 16 + ((wave & 1) ? 15 - lane : lane);
+      candv[slot] = kv;
+      candi[slot] = ki;
     }
     __syncthreads();
     if (wave == 0) {
       kv = candv[lane];
       ki = candi[lane];
-      bitonic64(kv, ki, lane);
+      bitonic_stage<32, 16>(kv, ki, lane);
+      bitonic_stage<64, 32>(kv, ki, lane);
       selk = ki;
     }
   } else {
