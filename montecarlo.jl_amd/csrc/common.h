@@ -92,12 +92,10 @@ void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths);
 hipError_t udt_check(const UdtBatch& b);
 hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s);
 hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s);
-hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode, hipStream_t s);
+hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, hipStream_t s);
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s);
 // Q accumulation into U (U preset to I) for every panel, in one launch
-hipError_t launch_udt_qbuild(const UdtBatch& b, hipStream_t s);
-// true when the Q build reads U and needs it set to the identity first (n > 256)
-bool udt_qbuild_needs_identity(int n);
+
 
 // rdivp: A <- A[:, pivot] * triu(T)^-1 with O as n*n scratch (qr.hip).
 struct RdivpItem {

@@ -377,24 +377,18 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
   {
     Timed t(h, DQMC_KERNEL_UDT);
     HIPCHK(h, launch_udt_norms(u, h->st));
+    for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
   }
   for (int pi = 0; pi < np; pi++) {
     {
       Timed t(h, DQMC_KERNEL_UDT_PANEL);
       HIPCHK(h, launch_udt_panel(u, st[pi], wd[pi], pi == np - 1, h->st));
     }
-    if (st[pi] + wd[pi] < h->n) {
+    {
+      // trailing update of A and Q accumulation on U's rows, one launch
       Timed t(h, DQMC_KERNEL_UDT_UPDATE);
-      HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], st[pi] + wd[pi], 0, h->st));
+      HIPCHK(h, launch_udt_reflect(u, st[pi], wd[pi], h->st));
     }
-  }
-  if (udt_qbuild_needs_identity(h->n)) {
-    Timed t(h, DQMC_KERNEL_UDT);
-    for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
-  }
-  {
-    Timed t(h, DQMC_KERNEL_UDT_UPDATE);
-    HIPCHK(h, launch_udt_qbuild(u, h->st));
   }
   Timed t(h, DQMC_KERNEL_UDT);
   HIPCHK(h, launch_udt_finish(u, h->st));

@@ -11,7 +11,9 @@
 //  * Blocked Householder with panels of w <= 16 columns factorised in LDS by one
 //    workgroup, and the trailing matrix updated with the compact-WY block reflector
 //    I - V T V^T by one wave per 16-column strip on the matrix cores, so the trailing
-//    matrix is streamed once per panel instead of once per column.
+//    matrix is streamed once per panel instead of once per column.  The same launch
+//    accumulates Q = H_1 ... H_P forward on 16-row strips of U (U <- U (I - V T V^T),
+//    starting from the identity), so Q needs no launches of its own.
 //  * Pivoting: at every panel start the trailing columns' residual norms are recomputed
 //    exactly and the w largest (first index wins ties) are moved to the front with the
 //    reference's sequential swap semantics.  Inside a panel the order is kept.  This is
@@ -61,12 +63,11 @@ __device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
 // The UDT runs as a sequence of launches per batch (one stream, so each sees the previous
 // one's results): column norms; per panel a single-workgroup panel kernel (pivot choice,
 // column moves, Householder factorisation of the panel in LDS, compact-WY factor T) and a
-// many-workgroup trailing update; Q by backward accumulation in one many-workgroup launch;
-// finally D and T.  Per-item state between launches: it.tau holds the residual
+// many-workgroup launch that applies the panel's block reflector to the trailing matrix
+// and accumulates Q forward on U's rows (U <- U H); finally D and T.  Per-item state between launches: it.tau holds the residual
 // norms, it.pivot the working permutation, it.tfac the T factor of each panel.
 
 // Rows of the panel per thread of the panel kernel (256 threads): 1, 2, 4 or 8.
-constexpr int kMaxUdtPanels = 512;   // panels of an n <= 2048 UDT (widths >= 8 beyond 256 rows)
 
 __host__ __device__ static int udt_rows_per_thread(int rows) {
   int R = 1;
@@ -664,12 +665,15 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
   const int rows = n - p0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const bool trans = mode == 0;
+  const bool trans = mode != 1;
   const double* Vp = it.A + (size_t)p0 * n + p0;
-  double* __restrict__ X = mode == 0 ? it.A : it.U;
   const int col = cb + li;
   const bool colok = col < n;
-  double* Xc = X + (size_t)(colok ? col : n - 1) * n + p0;
+  const int colc = colok ? col : n - 1;
+  // X[i][c] (i >= p0): column cb + c of A (mode 0) or of U (mode 1), or row cb + c of U
+  // (mode 2: X = U^T, so X -= V T^T V^T X is U <- U - U V T V^T on those rows)
+  const size_t xs = mode == 2 ? (size_t)n : 1;
+  double* Xc = mode == 2 ? it.U + colc + (size_t)p0 * n : (mode == 0 ? it.A : it.U) + (size_t)colc * n + p0;
   double ta[4];
 #pragma unroll
   for (int kc = 0; kc < 4; kc++) {
@@ -688,7 +692,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
       for (int kc = 0; kc < 4; kc++) {
         const int i = 16 * blk + 4 * kc + lk;
         const int ic = i < rows ? i : rows - 1;
-        const double x = Xc[ic];
+        const double x = Xc[ic * xs];
         xb[q][kc] = (i < rows && colok) ? x : 0.0;
         vb[q][kc] = vpanel(Vp, n, rows, w, i, li);
       }
@@ -726,7 +730,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
         for (int kc = 0; kc < 4; kc++) {
           const int i = 16 * blk + 4 * kc + lk;
           const int ic = i < rows ? i : rows - 1;
-          const double x = Xc[ic];
+          const double x = Xc[ic * xs];
           xb[q][kc] = (i < rows && colok) ? x : 0.0;
         }
       }
@@ -750,7 +754,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
         for (int r = 0; r < 4; r++) {
           const int i = 16 * blk + lk + 4 * r;
           if (i < rows && colok) {
-            Xc[i] = acc[r];
+            Xc[i * xs] = acc[r];
             if (i >= w) ss += acc[r] * acc[r];
           }
         }
@@ -767,133 +771,15 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
   }
 }
 
-__global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int mode) {
+// Panel p0's block reflector applied in one launch: workgroups [0, nA) update the trailing
+// 16-column strips of A from column c0 (with residual norms), the rest accumulate Q on
+// 16-row strips of U (U <- U H, forward: U holds H_1 ... H_k after panel k).
+__global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int nA) {
   __shared__ double red[4][4][64];
   __shared__ double nred[4][16];
-  reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * blockIdx.x, mode, red, nred);
-}
-
-// Entry (i, a) of panel p0's reflector block V for absolute row i (unit diagonal, zero
-// above it and beyond the w reflectors), read with an unconditional in-bounds load (the
-// diagonal element when the entry is not stored) and then selected.
-__device__ __forceinline__ double vabs(const double* A, int n, int p0, int w, int i, int a) {
-  const int ac = a < w ? a : 0;
-  const bool stored = a < w && i > p0 + a && i < n;
-  const double x = A[(size_t)(stored ? i : p0 + ac) + (size_t)(p0 + ac) * n];
-  return stored ? x : (a < w && i == p0 + a ? 1.0 : 0.0);
-}
-
-// Q = H_1 ... H_P by backward accumulation in one launch: the workgroup of U's columns
-// [16 s, 16 s + 16) applies the block reflectors of the panels from the last to the first.
-// Columns left of a panel's start are still unit columns there (zero in the reflector's
-// rows), so a strip is skipped only where it lies wholly left of the panel.
-// n <= 256: the strip starts as the identity in registers (absolute rows, the f64 MFMA
-// B/D layout: wave v, group q holds the 16-row block v + 4 q) and stays there until the
-// end; each panel's V and T are loaded while the previous panel is applied.  Larger n:
-// one reflect_strip pass per panel over U, which then holds the identity beforehand.
-__device__ __forceinline__ void qbuild_load(const UdtItem& it, int n, int p0, int w, int wave, int li, int lk,
-                                            double (&vb)[4][4], double (&va)[4][4], double (&ta)[4]) {
-#pragma unroll
-  for (int kc = 0; kc < 4; kc++) ta[kc] = it.tfac[(size_t)p0 * 16 + li * 16 + 4 * kc + lk];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int blk = wave + 4 * q;
-#pragma unroll
-    for (int kc = 0; kc < 4; kc++) {
-      vb[q][kc] = vabs(it.A, n, p0, w, 16 * blk + 4 * kc + lk, li);
-      va[q][kc] = vabs(it.A, n, p0, w, 16 * blk + li, 4 * kc + lk);
-    }
-  }
-}
-
-__device__ __forceinline__ void qbuild_apply(double (&xb)[4][4], const double (&vb)[4][4], const double (&va)[4][4],
-                                             const double (&ta)[4], double (&red)[4][4][64], int wave, int lane) {
-  dx4 W = dx4{0, 0, 0, 0};
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-#pragma unroll
-    for (int kc = 0; kc < 4; kc++) W = mfma16(vb[q][kc], xb[q][kc], W);
-#pragma unroll
-  for (int r = 0; r < 4; r++) red[wave][r][lane] = W[r];
-  __syncthreads();
-  dx4 Wt;
-#pragma unroll
-  for (int r = 0; r < 4; r++) Wt[r] = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
-  dx4 Y = dx4{0, 0, 0, 0};
-#pragma unroll
-  for (int kc = 0; kc < 4; kc++) Y = mfma16(ta[kc], Wt[kc], Y);
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    dx4 acc;
-#pragma unroll
-    for (int r = 0; r < 4; r++) acc[r] = xb[q][r];
-#pragma unroll
-    for (int kc = 0; kc < 4; kc++) acc = mfma16(-va[q][kc], Y[kc], acc);
-#pragma unroll
-    for (int r = 0; r < 4; r++) xb[q][r] = acc[r];
-  }
-  __syncthreads();                      // red is rewritten by the next panel
-}
-
-__host__ __device__ inline bool udt_qbuild_in_registers(int n) { return n <= 256; }
-
-__global__ __launch_bounds__(256) void udt_qbuild_kernel(UdtBatch batch) {
-  __shared__ double red[4][4][64];
-  __shared__ double nred[4][16];
-  __shared__ int pst[kMaxUdtPanels], pwd[kMaxUdtPanels];
-  __shared__ int np;
-  const int n = batch.n;
-  if (threadIdx.x == 0) {
-    int k = 0;
-    for (int p0 = 0; p0 < n && k < kMaxUdtPanels; k++) {
-      const int w = udt_panel_width(n - p0);
-      pst[k] = p0;
-      pwd[k] = w;
-      p0 += w;
-    }
-    np = k;
-  }
-  __syncthreads();
-  const int cb = 16 * blockIdx.x;
-  const UdtItem it = batch.it[blockIdx.y];
-  if (udt_qbuild_in_registers(n)) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int col = cb + li;
-    double xb[4][4];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int kc = 0; kc < 4; kc++) xb[q][kc] = 16 * (wave + 4 * q) + 4 * kc + lk == col ? 1.0 : 0.0;
-    int pmax = -1;                        // the panels applied: 0 .. pmax (starts below cb + 16)
-    for (int pi = 0; pi < np; pi++)
-      if (pst[pi] < cb + 16) pmax = pi;
-    double vb[4][4], va[4][4], ta[4], vb2[4][4], va2[4][4], ta2[4];
-    if (pmax >= 0) qbuild_load(it, n, pst[pmax], pwd[pmax], wave, li, lk, vb, va, ta);
-    // two panels per trip, so the prefetch buffers alternate without copies
-    for (int pi = pmax; pi >= 0; pi -= 2) {
-      if (pi >= 1) qbuild_load(it, n, pst[pi - 1], pwd[pi - 1], wave, li, lk, vb2, va2, ta2);
-      qbuild_apply(xb, vb, va, ta, red, wave, lane);
-      if (pi < 1) break;
-      if (pi >= 2) qbuild_load(it, n, pst[pi - 2], pwd[pi - 2], wave, li, lk, vb, va, ta);
-      qbuild_apply(xb, vb2, va2, ta2, red, wave, lane);
-    }
-    if (col < n) {
-#pragma unroll
-      for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int kc = 0; kc < 4; kc++) {
-          const int i = 16 * (wave + 4 * q) + 4 * kc + lk;
-          if (i < n) it.U[(size_t)i + (size_t)col * n] = xb[q][kc];
-        }
-    }
-    return;
-  }
-  for (int pi = np - 1; pi >= 0; pi--) {
-    if (cb + 16 <= pst[pi]) continue;     // uniform across the workgroup
-    reflect_strip(it, n, pst[pi], pwd[pi], cb, 1, red, nred);
-    __syncthreads();                      // red is reused by the next panel
-  }
+  const int bx = blockIdx.x;
+  if (bx < nA) reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * bx, 0, red, nred);
+  else reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, 16 * (bx - nA), 2, red, nred);
 }
 
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
@@ -969,16 +855,11 @@ hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStrea
   return hipGetLastError();
 }
 
-hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, int c0, int mode, hipStream_t s) {
-  if (c0 >= b.n) return hipSuccess;
-  hipLaunchKernelGGL(udt_reflect_kernel, dim3((b.n - c0 + 15) / 16, b.count), dim3(256), 0, s, b, p0, w, c0, mode);
-  return hipGetLastError();
-}
-
-bool udt_qbuild_needs_identity(int n) { return !udt_qbuild_in_registers(n); }
-
-hipError_t launch_udt_qbuild(const UdtBatch& b, hipStream_t s) {
-  hipLaunchKernelGGL(udt_qbuild_kernel, dim3((b.n + 15) / 16, b.count), dim3(256), 0, s, b);
+hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, hipStream_t s) {
+  const int c0 = p0 + w;
+  const int nA = c0 < b.n ? (b.n - c0 + 15) / 16 : 0;
+  const int nQ = (b.n + 15) / 16;
+  hipLaunchKernelGGL(udt_reflect_kernel, dim3(nA + nQ, b.count), dim3(256), 0, s, b, p0, w, c0, nA);
   return hipGetLastError();
 }
 
@@ -996,14 +877,12 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
   udt_plan(b.n, st, wd);
   const int np = (int)st.size();
   if ((e = launch_udt_norms(b, s)) != hipSuccess) return e;
+  for (int k = 0; k < b.count; k++)
+    if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
   for (int pi = 0; pi < np; pi++) {
     if ((e = launch_udt_panel(b, st[pi], wd[pi], pi == np - 1, s)) != hipSuccess) return e;
-    if ((e = launch_udt_reflect(b, st[pi], wd[pi], st[pi] + wd[pi], 0, s)) != hipSuccess) return e;
+    if ((e = launch_udt_reflect(b, st[pi], wd[pi], s)) != hipSuccess) return e;
   }
-  if (!udt_qbuild_in_registers(b.n))
-    for (int k = 0; k < b.count; k++)
-      if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
-  if ((e = launch_udt_qbuild(b, s)) != hipSuccess) return e;
   return launch_udt_finish(b, s);
 }
 
