@@ -377,7 +377,6 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
   {
     Timed t(h, DQMC_KERNEL_UDT);
     HIPCHK(h, launch_udt_norms(u, h->st));
-    for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_set_identity(U[b], h->n, 1, h->st));
   }
   for (int pi = 0; pi < np; pi++) {
     {

@@ -13,7 +13,8 @@
 //    I - V T V^T by one wave per 16-column strip on the matrix cores, so the trailing
 //    matrix is streamed once per panel instead of once per column.  The same launch
 //    accumulates Q = H_1 ... H_P forward on 16-row strips of U (U <- U (I - V T V^T),
-//    starting from the identity), so Q needs no launches of its own.
+//    the first panel's launch starting from the identity without reading U), so Q needs
+//    no launches of its own.
 //  * Pivoting: at every panel start the trailing columns' residual norms are recomputed
 //    exactly and the w largest (first index wins ties) are moved to the front with the
 //    reference's sequential swap semantics.  Inside a panel the order is kept.  This is
@@ -674,6 +675,9 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
   // (mode 2: X = U^T, so X -= V T^T V^T X is U <- U - U V T V^T on those rows)
   const size_t xs = mode == 2 ? (size_t)n : 1;
   double* Xc = mode == 2 ? it.U + colc + (size_t)p0 * n : (mode == 0 ? it.A : it.U) + (size_t)colc * n + p0;
+  // the first panel's Q strips start from the identity (U is not read; every column of the
+  // strip is written)
+  const bool ident = mode == 2 && p0 == 0;
   double ta[4];
 #pragma unroll
   for (int kc = 0; kc < 4; kc++) {
@@ -692,7 +696,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
       for (int kc = 0; kc < 4; kc++) {
         const int i = 16 * blk + 4 * kc + lk;
         const int ic = i < rows ? i : rows - 1;
-        const double x = Xc[ic * xs];
+        const double x = ident ? (ic == colc ? 1.0 : 0.0) : Xc[ic * xs];
         xb[q][kc] = (i < rows && colok) ? x : 0.0;
         vb[q][kc] = vpanel(Vp, n, rows, w, i, li);
       }
@@ -730,7 +734,7 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
         for (int kc = 0; kc < 4; kc++) {
           const int i = 16 * blk + 4 * kc + lk;
           const int ic = i < rows ? i : rows - 1;
-          const double x = Xc[ic * xs];
+          const double x = ident ? (ic == colc ? 1.0 : 0.0) : Xc[ic * xs];
           xb[q][kc] = (i < rows && colok) ? x : 0.0;
         }
       }
@@ -877,8 +881,6 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
   udt_plan(b.n, st, wd);
   const int np = (int)st.size();
   if ((e = launch_udt_norms(b, s)) != hipSuccess) return e;
-  for (int k = 0; k < b.count; k++)
-    if ((e = launch_set_identity(b.it[k].U, b.n, 1, s)) != hipSuccess) return e;
   for (int pi = 0; pi < np; pi++) {
     if ((e = launch_udt_panel(b, st[pi], wd[pi], pi == np - 1, s)) != hipSuccess) return e;
     if ((e = launch_udt_reflect(b, st[pi], wd[pi], s)) != hipSuccess) return e;
