@@ -829,9 +829,11 @@ void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths) {
 
 hipError_t udt_check(const UdtBatch& b) {
   if (b.count < 0 || b.count > kMaxUdtItems || b.n < 1 || b.n > 2048) return hipErrorInvalidValue;
-  if (b.apply_pivot)
-    for (int k = 0; k < b.count; k++)
-      if (b.it[k].T == b.it[k].A) return hipErrorInvalidValue;  // pivoted T is written out of place
+  for (int k = 0; k < b.count; k++) {
+    // U is written while A's reflectors are read: it must be its own buffer
+    if (b.it[k].U == b.it[k].A || b.it[k].U == b.it[k].T) return hipErrorInvalidValue;
+    if (b.apply_pivot && b.it[k].T == b.it[k].A) return hipErrorInvalidValue;  // pivoted T is written out of place
+  }
   return hipSuccess;
 }
 
