@@ -34,9 +34,17 @@ extern "C" {
 #define DQMC_E_HIP (-2)
 #define DQMC_E_STATE (-3)
 #define DQMC_E_ALLOC (-4)
+#define DQMC_E_DEVICE (-5) /* a kernel gave up a bounded wait or met a broken invariant:
+                              the chain's G is not valid (count in dqmc_stats_t.device_errors) */
+
+#define DQMC_E_PRECISION (-6) /* mixed precision: a stabilisation found max|G_wrapped - G_scratch|
+                                 above dqmc_config_t.mixed_tolerance (count and largest value
+                                 in dqmc_stats_t; G was replaced by the FP64 recompute)     */
 
 #define DQMC_PRECISION_FP64 0  /* everything in FP64 */
-#define DQMC_PRECISION_MIXED 1 /* FP32 wraps and rank-1 updates; FP64 UDT, recompute, detratio */
+#define DQMC_PRECISION_MIXED 1 /* FP32 wraps and rank-1 updates between stabilisations; FP64
+                                  UDT, recompute and detratio.  Every stabilisation checks
+                                  max|G_wrapped - G_scratch| against mixed_tolerance.        */
 
 typedef struct dqmc_chain_s* dqmc_chain_t;
 
@@ -52,6 +60,8 @@ typedef struct {
   int delay_depth;  /* sites per local-update window: <= 16 -> 16, <= 32 -> 32, else 64;
                        0 = automatic (32)                                              */
   int device;       /* HIP device ordinal                                                   */
+  double mixed_tolerance; /* DQMC_PRECISION_MIXED: bound on max|G_wrapped - G_scratch| at each
+                             stabilisation; 0 = 1e-6.  Ignored in FP64.                    */
 } dqmc_config_t;
 
 typedef struct {
@@ -67,6 +77,10 @@ typedef struct {
   int64_t delayed_flushes;      /* rank-k flushes of the delayed update                      */
   double propagation_error_max; /* max over stabilisations of max|G_wrapped - G_scratch|    */
   double propagation_error_last;
+  int64_t device_errors;        /* kernels that gave up a bounded wait or met a broken
+                                   invariant (each makes the call that sees it return
+                                   DQMC_E_DEVICE)                                          */
+  int64_t precision_violations; /* mixed precision: stabilisations above mixed_tolerance    */
 } dqmc_stats_t;
 
 /* DQMCStack(field, model, checkerboard=false) + initialize_stack (stack.jl:113-206).
@@ -136,8 +150,10 @@ int dqmc_batch_sweep(dqmc_chain_t* chains, int count, int nsweeps, int64_t* acce
  *     while others run are admitted as slots and memory free up, and run concurrently, each
  *     on its own host thread and HIP stream on `device`.
  *   - A result is the same as the chain run alone (its kernels touch only its own buffers).
- *   - The server's entry points are called from one host thread; dqmc_server_last_error(s)
- *     describes the last failure (s = NULL: the last creation failure of this thread).
+ *   - The server's entry points may be called from several host threads;
+ *     dqmc_server_last_error describes the last failure of the calling thread.
+ *   - dqmc_server_destroy marks queued requests failed, waits for running ones and for
+ *     threads inside dqmc_server_wait to return, then frees the server.
  *   - request_state: 0 queued, 1 running, 2 done, 3 failed.                              */
 typedef struct dqmc_server_s* dqmc_server_t;
 typedef struct {
@@ -188,6 +204,11 @@ int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* G, int ld);
 
 int dqmc_get_state(dqmc_chain_t h, int* current_slice, int* direction, int* current_range);
 int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* stats);
+/* Bound of the window kernel's internal wait (polls of an LDS progress counter, default
+ * 2^22, about 0.1 s): a wave that reaches it leaves its loop, the launch still ends, and
+ * the event is a device error.  No reference counterpart (the reference's update is
+ * sequential host code).                                                               */
+int dqmc_set_wait_limit(dqmc_chain_t h, int64_t polls);
 /* max|G_wrapped - G_scratch| at every stabilisation since the last call (mixed precision:
  * FP32-propagated G against the FP64 recompute).                                       */
 int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, int64_t* length);

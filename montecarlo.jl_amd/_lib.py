@@ -24,7 +24,7 @@ class Config(C.Structure):
     _fields_ = [("n_sites", C.c_int), ("flavors", C.c_int), ("slices", C.c_int),
                 ("safe_mult", C.c_int), ("delta_tau", C.c_double), ("alpha", C.c_double),
                 ("precision", C.c_int), ("check_propagation_error", C.c_int),
-                ("delay_depth", C.c_int), ("device", C.c_int)]
+                ("delay_depth", C.c_int), ("device", C.c_int), ("mixed_tolerance", C.c_double)]
 
 
 class ServerConfig(C.Structure):
@@ -43,7 +43,8 @@ class Stats(C.Structure):
                 ("stream_exhausted", C.c_int64), ("propagation_error_count", C.c_int64),
                 ("stabilizations", C.c_int64), ("sweeps", C.c_int64),
                 ("delayed_flushes", C.c_int64), ("propagation_error_max", C.c_double),
-                ("propagation_error_last", C.c_double)]
+                ("propagation_error_last", C.c_double), ("device_errors", C.c_int64),
+                ("precision_violations", C.c_int64)]
 
 
 EXPORTED = [
@@ -54,7 +55,7 @@ EXPORTED = [
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
-    "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats",
+    "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats", "dqmc_set_wait_limit",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
     "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
@@ -106,6 +107,7 @@ def load():
         "dqmc_calculate_greens": [vp, i, _dp, i],
         "dqmc_get_state": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
         "dqmc_get_stats": [vp, C.POINTER(Stats)],
+        "dqmc_set_wait_limit": [vp, i64],
         "dqmc_get_stabilization_log": [vp, vp, i64, C.POINTER(i64)],
         "dqmc_synchronize": [vp],
         "dqmc_la_udt": [i, _dp, i, _dp, _dp, _dp, _i32],
@@ -247,12 +249,12 @@ class HIPChain:
 
     def __init__(self, n: int, flv: int, slices: int, safe_mult: int, delta_tau: float,
                  alpha: float, check_propagation_error: bool = True, precision: str = "fp64",
-                 delay_depth: int = 0, device: int = 0):
+                 delay_depth: int = 0, device: int = 0, mixed_tolerance: float = 0.0):
         L = load()
         self.n, self.flv, self.M, self.safe_mult = n, flv, slices, safe_mult
         self.delta_tau, self.alpha, self.precision = delta_tau, alpha, precision
         cfg = Config(n, flv, slices, safe_mult, delta_tau, alpha, PRECISION[precision],
-                     int(check_propagation_error), delay_depth, device)
+                     int(check_propagation_error), delay_depth, device, mixed_tolerance)
         h = C.c_void_p()
         _check(L.dqmc_create(C.byref(cfg), C.byref(h)), None, "dqmc_create")
         self._h = h
@@ -396,7 +398,12 @@ class HIPChain:
                 "ties": s["ties"], "prop_err_count": s["propagation_error_count"],
                 "stabilizations": s["stabilizations"], "stream_exhausted": s["stream_exhausted"],
                 "prop_err_max": s["propagation_error_max"],
-                "prop_err_last": s["propagation_error_last"]}
+                "prop_err_last": s["propagation_error_last"], "device_errors": s["device_errors"],
+                "precision_violations": s["precision_violations"]}
+
+    def set_wait_limit(self, polls: int):
+        """Bound of the window kernel's internal wait (dqmc_set_wait_limit)."""
+        self._c(load().dqmc_set_wait_limit(self._h, polls), "dqmc_set_wait_limit")
 
     def stabilization_log(self) -> np.ndarray:
         length = C.c_int64()

@@ -82,6 +82,7 @@ struct UdtBatch {
   int count;
   int n;
   int apply_pivot;
+  unsigned long long* err;   // device error counter (may be null): +1 per broken pivot selection
 };
 hipError_t launch_udt(const UdtBatch& b, hipStream_t s);
 // The steps of launch_udt, for callers that time them separately: panel starts and widths,
@@ -150,6 +151,7 @@ struct ChainCounters {
   long long exhausted;
   long long log_len;
   long long flushes;
+  long long device_errors;  // kernels that gave up a bounded wait or met a broken invariant
 };
 
 // Window kernel of the local update (local.hip): sites [w0, w0 + W) of one slice.  The
@@ -180,6 +182,7 @@ struct WindowArgs {
   unsigned char* dec_log;
   double* p_log;
   long long log_cap;
+  long long wait_limit;   // polls before a factor wave gives up (counted in ctr->device_errors)
 };
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);

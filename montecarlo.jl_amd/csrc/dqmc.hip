@@ -77,6 +77,11 @@ struct dqmc_chain_s {
   double* errbuf = nullptr;
   long long errcap = 4096, nerr = 0;
   double* scal = nullptr;  // small device scalars
+  long long wait_limit = 1ll << 22;  // polls of the window kernel's factor waves before giving up
+  long long dev_err_seen = 0;        // device errors already reported
+  double mixed_tol = 1e-6;           // mixed precision: bound on max|G_wrapped - G_scratch|
+  long long prec_viol = 0, prec_viol_seen = 0;
+  double prec_viol_max = 0.0;
 
   int cs = 0, cr = 1, dir = 1;
   bool hopping_set = false;
@@ -363,6 +368,7 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
   u.count = h->flv;
   u.n = h->n;
   u.apply_pivot = apply_pivot;
+  u.err = h->ctr ? reinterpret_cast<unsigned long long*>(&h->ctr->device_errors) : nullptr;
   for (int b = 0; b < h->flv; b++) {
     u.it[b].A = A[b]; u.it[b].U = U[b]; u.it[b].D = D[b]; u.it[b].T = T[b];
     u.it[b].pivot = h->piv + (size_t)b * h->n;
@@ -516,22 +522,45 @@ int to32(Chain* h) {
   return DQMC_OK;
 }
 
+// One stabilisation's max|G_wrapped - G_scratch|: propagation-error statistics (the
+// reference's 1e-7 check, stack.jl:644-654) and, in mixed precision, the bound check.
+void absorb_error(Chain* h, double x) {
+  h->stabs++;
+  h->perr_last = x;
+  h->stablog.push_back(x);
+  if (x > 1e-7 || x != x) { h->perr_count++; if (x > h->perr_max || x != x) h->perr_max = x; }
+  if (h->prec == DQMC_PRECISION_MIXED && (x > h->mixed_tol || x != x)) {
+    h->prec_viol++;
+    if (x > h->prec_viol_max || x != x) h->prec_viol_max = x;
+  }
+}
+
 int record_error(Chain* h, const double* a, const float* a32, const double* b) {
   if (h->nerr >= h->errcap) {
     // harvest pending errors before reusing the device slots
     std::vector<double> v(h->nerr);
     HIPCHK(h, hipMemcpyAsync(v.data(), h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
-    for (double x : v) {
-      h->stabs++;
-      h->perr_last = x;
-      h->stablog.push_back(x);
-      if (x > 1e-7 || x != x) { h->perr_count++; if (x > h->perr_max || x != x) h->perr_max = x; }
-    }
+    for (double x : v) absorb_error(h, x);
     h->nerr = 0;
   }
   HIPCHK(h, launch_maxabsdiff(a, a32, b, (int64_t)h->flv * h->nn, h->errbuf + h->nerr, h->st));
   h->nerr++;
+  return DQMC_OK;
+}
+
+// Device errors (a bounded wait that gave up, a broken pivot selection) leave the chain's
+// G wrong: every call that waits for the device reports new ones as DQMC_E_DEVICE.
+int check_device_errors(Chain* h) {
+  if (!h->ctr) return DQMC_OK;
+  long long e = 0;
+  HIPCHK(h, hipMemcpy(&e, &h->ctr->device_errors, sizeof(e), hipMemcpyDeviceToHost));
+  if (e > h->dev_err_seen) {
+    h->err = "device error: " + std::to_string(e - h->dev_err_seen) +
+             " kernel(s) gave up a bounded wait or met a broken pivot selection; G is not valid";
+    h->dev_err_seen = e;
+    return DQMC_E_DEVICE;
+  }
   return DQMC_OK;
 }
 
@@ -541,13 +570,19 @@ int harvest(Chain* h) {
   if (h->nerr > 0) {
     std::vector<double> v(h->nerr);
     HIPCHK(h, hipMemcpy(v.data(), h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost));
-    for (double x : v) {
-      h->stabs++;
-      h->perr_last = x;
-      h->stablog.push_back(x);
-      if (x > 1e-7 || x != x) { h->perr_count++; if (x > h->perr_max || x != x) h->perr_max = x; }
-    }
+    for (double x : v) absorb_error(h, x);
     h->nerr = 0;
+  }
+  CHK(check_device_errors(h));
+  if (h->prec_viol > h->prec_viol_seen) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "mixed precision: %lld stabilisation(s) with max|G_wrapped - G_scratch| above the "
+                  "tolerance %.3g (largest %.3g); G was replaced by the FP64 recompute each time",
+                  h->prec_viol - h->prec_viol_seen, h->mixed_tol, h->prec_viol_max);
+    h->err = buf;
+    h->prec_viol_seen = h->prec_viol;
+    return DQMC_E_PRECISION;
   }
   return DQMC_OK;
 }
@@ -653,6 +688,7 @@ int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
   for (int k = 0; k < 2; k++) { a.d_first[k] = h->d_first[k]; a.d_second[k] = h->d_second[k]; a.bweight[k] = h->bweight[k]; }
   a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
   a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
+  a.wait_limit = h->wait_limit;
   int slot = 0;
   auto launch = [&](int kind) -> int {
     Timed t(h, kind);
@@ -781,9 +817,11 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   *out = nullptr;
   if (cfg->n_sites < 1 || cfg->n_sites > 2048 || (cfg->flavors != 1 && cfg->flavors != 2) ||
       cfg->slices < 1 || cfg->safe_mult < 1 || !(cfg->delta_tau > 0) ||
-      (cfg->precision != DQMC_PRECISION_FP64 && cfg->precision != DQMC_PRECISION_MIXED)) {
+      (cfg->precision != DQMC_PRECISION_FP64 && cfg->precision != DQMC_PRECISION_MIXED) ||
+      !(cfg->mixed_tolerance >= 0)) {
     g_create_error = "dqmc_create: invalid configuration (1 <= n_sites <= 2048, flavors 1|2, "
-                     "slices >= 1, safe_mult >= 1, delta_tau > 0, precision 0|1)";
+                     "slices >= 1, safe_mult >= 1, delta_tau > 0, precision 0|1, "
+                     "mixed_tolerance >= 0)";
     return DQMC_E_INVALID;
   }
   hipError_t e = hipSetDevice(cfg->device);
@@ -798,6 +836,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   h->n = cfg->n_sites; h->flv = cfg->flavors; h->M = cfg->slices; h->sm = cfg->safe_mult;
   h->dtau = cfg->delta_tau; h->alpha = cfg->alpha; h->prec = cfg->precision;
   h->check = cfg->check_propagation_error ? 1 : 0;
+  if (cfg->mixed_tolerance > 0) h->mixed_tol = cfg->mixed_tolerance;
   h->K = generate_chunks(h->M, h->sm, h->rs, h->re);
   h->nn = (size_t)h->n * h->n;
   const double a = h->alpha;
@@ -1255,7 +1294,9 @@ int dqmc_get_state(dqmc_chain_t h, int* cs, int* dir, int* cr) {
 int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* s) {
   ENTER(h);
   if (!s) return DQMC_E_INVALID;
-  CHK(harvest(h));
+  const int hr = harvest(h);
+  // device errors and precision violations are reported in the stats themselves
+  if (hr != DQMC_OK && hr != DQMC_E_DEVICE && hr != DQMC_E_PRECISION) return hr;
   ChainCounters c;
   HIPCHK(h, hipMemcpy(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost));
   s->accepted = c.accepted;
@@ -1270,6 +1311,15 @@ int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* s) {
   s->delayed_flushes = c.flushes;
   s->propagation_error_max = h->perr_max;
   s->propagation_error_last = h->perr_last;
+  s->device_errors = c.device_errors;
+  s->precision_violations = h->prec_viol;
+  return DQMC_OK;
+}
+
+int dqmc_set_wait_limit(dqmc_chain_t h, int64_t polls) {
+  ENTER(h);
+  if (polls < 0) { h->err = "dqmc_set_wait_limit: polls < 0"; return DQMC_E_INVALID; }
+  h->wait_limit = polls;
   return DQMC_OK;
 }
 
@@ -1329,9 +1379,11 @@ int la_make(LaCtx& x, int n, int nbuf) {
     A(&h->tfac, (size_t)(n + 16) * 16); A(&h->swp, (size_t)n * 32);
     A(&h->Ul, h->nn); A(&h->Ur, h->nn); A(&h->Tl, h->nn); A(&h->Tr, h->nn); A(&h->Dr, (size_t)n);
     A(&h->G, h->nn);
+    A(&h->ctr, 1);
     return r;
   });
   (void)nbuf;
+  if (r == DQMC_OK && hipMemsetAsync(h->ctr, 0, sizeof(ChainCounters), h->st) != hipSuccess) r = DQMC_E_HIP;
   return r;
 }
 void la_free(LaCtx& x) {
@@ -1355,6 +1407,7 @@ int dqmc_la_udt(int n, const double* X, int apply_pivot, double* U, double* D, d
     double* Tout = apply_pivot ? h->tmp2 : h->tmp1;
     if (r == DQMC_OK) r = udt(h, B(h, h->tmp1).get(), B(h, h->cur).get(), Bd(h, h->Dl).get(), B(h, Tout).get(), apply_pivot);
     if (r == DQMC_OK && hipStreamSynchronize(h->st) != hipSuccess) r = DQMC_E_HIP;
+    if (r == DQMC_OK) r = check_device_errors(h);
     if (r == DQMC_OK) {
       if (hipMemcpy(U, h->cur, bytes, hipMemcpyDeviceToHost) != hipSuccess ||
           hipMemcpy(T, Tout, bytes, hipMemcpyDeviceToHost) != hipSuccess ||

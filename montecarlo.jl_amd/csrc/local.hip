@@ -500,14 +500,18 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       //           Binv[:, k] = Binv_(k) N[:, k], Binv[k][k] = 1; lane r holds row r.
       // Flip k needs only flips < k and its own site and gamma, all published before it.
       // The wait is bounded, so the wave always leaves the loop.
+      // Giving up leaves Y wrong, so it is counted as a device error that the host reports.
       int k = 0;
-      long spins = 0;
+      long long spins = 0;
       for (;;) {
         int avail = __hip_atomic_load(flags, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (avail <= k) {
           const int fin = __hip_atomic_load(flags + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (fin < 0) {
-            if (++spins > (1l << 22)) break;
+            if (++spins > a.wait_limit) {
+              if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&a.ctr->device_errors), 1ull);
+              break;
+            }
             __builtin_amdgcn_s_sleep(1);
             continue;
           }

@@ -438,8 +438,9 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     for (int t = 0; t < w; t++) {
       const int c = __builtin_amdgcn_readlane(selc, t);
       const unsigned long long hold = __ballot(slot >= 0 && cont == c);
-      // every choice is held by exactly one tracked slot; an empty ballot would mean a
-      // broken selection, and then the swap is skipped
+      // every choice is held by exactly one tracked slot; an empty ballot means a broken
+      // selection: the swap is skipped and the event counted as a device error
+      if (!hold && lane == 0 && batch.err) atomicAdd(batch.err, 1ull);
       const int src = hold ? __ffsll((long long)hold) - 1 : t;  // lane whose slot holds c
       const int a_cont = __builtin_amdgcn_readlane(cont, t);      // content of slot p0+t
       const int b_cont = __builtin_amdgcn_readlane(cont, src);

@@ -61,13 +61,15 @@ struct dqmc_server_s {
   int64_t used_bytes = 0;
   int active = 0;
   int peak_active = 0;
+  int waiters = 0;        // threads inside dqmc_server_wait (destroy waits for them to leave)
   bool stop = false;
   std::thread scheduler;
-  std::string err;
 };
 
 namespace {
 
+// Errors are kept per calling thread, so client threads that call the server concurrently
+// never write the same string.
 thread_local std::string t_server_error;
 
 // One request, start to finish, on the calling (worker) thread; false on failure (r.err).
@@ -134,7 +136,8 @@ void scheduler_loop(dqmc_server_s* s) {
 extern "C" {
 
 const char* dqmc_server_last_error(dqmc_server_t s) {
-  return s ? s->err.c_str() : t_server_error.c_str();
+  (void)s;
+  return t_server_error.c_str();
 }
 
 int dqmc_server_create(const dqmc_server_config_t* cfg, dqmc_server_t* out) {
@@ -155,16 +158,16 @@ int dqmc_server_submit(dqmc_server_t s, const dqmc_request_t* rq, int64_t* id) {
   const dqmc_config_t& c = rq->chain;
   int64_t fp = 0;
   if (dqmc_footprint_bytes(&c, &fp) != DQMC_OK) {
-    s->err = std::string("dqmc_server_submit: ") + dqmc_last_error(nullptr);
+    t_server_error = std::string("dqmc_server_submit: ") + dqmc_last_error(nullptr);
     return DQMC_E_INVALID;
   }
   const int n = c.n_sites;
   if (!rq->hopping || !rq->field || rq->ld_hopping < n || rq->ld_field < n || rq->nsweeps < 0) {
-    s->err = "dqmc_server_submit: null hopping/field, ld < n_sites or nsweeps < 0";
+    t_server_error = "dqmc_server_submit: null hopping/field, ld < n_sites or nsweeps < 0";
     return DQMC_E_INVALID;
   }
   if (fp > s->cfg.memory_budget) {
-    s->err = "dqmc_server_submit: the chain's footprint exceeds the server's memory budget";
+    t_server_error = "dqmc_server_submit: the chain's footprint exceeds the server's memory budget";
     return DQMC_E_ALLOC;
   }
   auto r = std::make_shared<Request>();
@@ -179,7 +182,7 @@ int dqmc_server_submit(dqmc_server_t s, const dqmc_request_t* rq, int64_t* id) {
   r->nsweeps = rq->nsweeps;
   r->footprint = fp;
   std::lock_guard<std::mutex> g(s->mu);
-  if (s->stop) { s->err = "dqmc_server_submit: server is shutting down"; return DQMC_E_STATE; }
+  if (s->stop) { t_server_error = "dqmc_server_submit: server is shutting down"; return DQMC_E_STATE; }
   r->id = s->next_id++;
   r->t_submit = now_s();
   s->reqs[r->id] = r;
@@ -193,7 +196,7 @@ int dqmc_server_poll(dqmc_server_t s, int64_t id, int* state) {
   if (!s || !state) return DQMC_E_INVALID;
   std::lock_guard<std::mutex> g(s->mu);
   auto it = s->reqs.find(id);
-  if (it == s->reqs.end()) { s->err = "dqmc_server_poll: unknown request id"; return DQMC_E_INVALID; }
+  if (it == s->reqs.end()) { t_server_error = "dqmc_server_poll: unknown request id"; return DQMC_E_INVALID; }
   *state = it->second->state;
   return DQMC_OK;
 }
@@ -202,9 +205,12 @@ int dqmc_server_wait(dqmc_server_t s, int64_t id) {
   if (!s) return DQMC_E_INVALID;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(id);
-  if (it == s->reqs.end()) { s->err = "dqmc_server_wait: unknown request id"; return DQMC_E_INVALID; }
+  if (it == s->reqs.end()) { t_server_error = "dqmc_server_wait: unknown request id"; return DQMC_E_INVALID; }
   std::shared_ptr<Request> r = it->second;
+  s->waiters++;
   s->cv.wait(lk, [&]() { return r->t_done > 0; });
+  s->waiters--;
+  s->cv.notify_all();
   return DQMC_OK;
 }
 
@@ -215,20 +221,20 @@ int dqmc_server_result(dqmc_server_t s, int64_t id, int8_t* conf, int ld_conf, d
   {
     std::lock_guard<std::mutex> g(s->mu);
     auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) { s->err = "dqmc_server_result: unknown request id"; return DQMC_E_INVALID; }
+    if (it == s->reqs.end()) { t_server_error = "dqmc_server_result: unknown request id"; return DQMC_E_INVALID; }
     r = it->second;
-    if (r->t_done <= 0) { s->err = "dqmc_server_result: request not finished"; return DQMC_E_STATE; }
+    if (r->t_done <= 0) { t_server_error = "dqmc_server_result: request not finished"; return DQMC_E_STATE; }
     s->reqs.erase(it);
   }
-  if (r->state == kFailed) { s->err = r->err; return DQMC_E_STATE; }
+  if (r->state == kFailed) { t_server_error = r->err; return DQMC_E_STATE; }
   const int n = r->cfg.n_sites;
   if (conf) {
-    if (ld_conf < n) { s->err = "dqmc_server_result: ld_conf < n_sites"; return DQMC_E_INVALID; }
+    if (ld_conf < n) { t_server_error = "dqmc_server_result: ld_conf < n_sites"; return DQMC_E_INVALID; }
     for (int l = 0; l < r->cfg.slices; l++)
       for (int i = 0; i < n; i++) conf[(size_t)i + (size_t)l * ld_conf] = r->conf_out[(size_t)i + (size_t)l * n];
   }
   if (G) {
-    if (ld_G < n) { s->err = "dqmc_server_result: ld_G < n_sites"; return DQMC_E_INVALID; }
+    if (ld_G < n) { t_server_error = "dqmc_server_result: ld_G < n_sites"; return DQMC_E_INVALID; }
     for (int j = 0; j < n * r->cfg.flavors; j++)
       for (int i = 0; i < n; i++) G[(size_t)i + (size_t)j * ld_G] = r->G_out[(size_t)i + (size_t)j * n];
   }
@@ -255,11 +261,18 @@ int dqmc_server_destroy(dqmc_server_t s) {
   if (!s) return DQMC_E_INVALID;
   {
     std::unique_lock<std::mutex> lk(s->mu);
-    // queued requests are dropped; running ones finish first
+    // queued requests are dropped (marked failed, so their waiters return); running ones
+    // finish first, and every thread inside dqmc_server_wait leaves before the server is freed
+    const double t = now_s();
+    for (auto& r : s->queue) {
+      r->state = kFailed;
+      r->err = "dropped: the server was destroyed before the request was admitted";
+      r->t_done = t;
+    }
     s->queue.clear();
     s->stop = true;
     s->cv.notify_all();
-    s->cv.wait(lk, [&]() { return s->active == 0; });
+    s->cv.wait(lk, [&]() { return s->active == 0 && s->waiters == 0; });
   }
   s->scheduler.join();
   delete s;

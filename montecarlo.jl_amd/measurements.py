@@ -85,3 +85,15 @@ class MeasurementSet:
 
     def mean(self, key: str):
         return self.sums[key] / max(self.count, 1)
+
+    def state(self) -> dict:
+        """Arrays that restore the running means (checkpoints)."""
+        out = {"count": np.asarray(self.count)}
+        for k, v in self.sums.items():
+            out["sum/" + k] = np.asarray(v)
+        return out
+
+    def restore(self, state: dict):
+        self.count = int(state["count"])
+        self.sums = {k[4:]: (float(v) if np.ndim(v) == 0 else np.array(v))
+                     for k, v in state.items() if k.startswith("sum/")}

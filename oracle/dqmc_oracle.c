@@ -541,6 +541,9 @@ EXPORT void orc_get_field(oracle_t *o, int8_t *conf) { memcpy(conf, o->conf, (si
 EXPORT void orc_set_uniforms(oracle_t *o, const double *u, int64_t count) {
     o->uniforms = u; o->n_uniforms = count; o->u_pos = 0; o->stream_exhausted = 0;
 }
+EXPORT void orc_set_seed_at(oracle_t *o, uint64_t seed, uint64_t pos) {
+    o->uniforms = NULL; o->n_uniforms = 0; o->seed = seed; o->u_pos = (int64_t)pos;
+}
 EXPORT void orc_set_seed(oracle_t *o, uint64_t seed) {
     o->uniforms = NULL; o->n_uniforms = 0; o->seed = seed; o->u_pos = 0;
 }

@@ -48,6 +48,7 @@ def lib():
         L.orc_get_field.argtypes = [vp, _i8]
         L.orc_set_uniforms.argtypes = [vp, C.POINTER(C.c_double), C.c_int64]
         L.orc_set_seed.argtypes = [vp, C.c_uint64]
+        L.orc_set_seed_at.argtypes = [vp, C.c_uint64, C.c_uint64]
         L.orc_set_log.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_int64]
         L.orc_build_stack_forward.argtypes = [vp]
         L.orc_reverse_build_stack.argtypes = [vp]
@@ -176,8 +177,8 @@ class OracleChain:
         self._keep = [u]
         lib().orc_set_uniforms(self._h, u.ctypes.data_as(C.POINTER(C.c_double)), len(u))
 
-    def set_seed(self, seed: int):
-        lib().orc_set_seed(self._h, C.c_uint64(seed))
+    def set_seed(self, seed: int, position: int = 0):
+        lib().orc_set_seed_at(self._h, C.c_uint64(seed), C.c_uint64(position))
 
     def set_log(self, cap: int):
         self._dec = np.zeros(cap, dtype=np.uint8)
