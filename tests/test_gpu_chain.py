@@ -2,13 +2,14 @@
 
 Reference path: reverse_build_stack/propagate (stack.jl:284-308, 605-730), calculate_greens
 (stack.jl:525-583), sweep_spatial (local_updates.jl:23-60).
-Tolerances: G within 1e-10 relative (max-norm) in FP64; accept sequence and field bit-exact
-except proposals whose |u - p| is within the tie band (counted by both sides).
+Tolerances: G within 1e-10 relative (max-norm) in FP64; the accept sequence is bit-exact up
+to the first proposal whose |u - p| lies within the tie band (1e-12), and the field and G
+after the sweep are compared in full when there is no tie.
 """
 import numpy as np
 import pytest
 
-from conftest import make_chain, rel_maxdiff
+from conftest import assert_same_decisions, counter_stream, make_chain, rel_maxdiff
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +89,7 @@ def test_one_sweep_accept_sequence(pkg, hip, oracle, L, U, beta, mu, stream):
         g.set_uniforms(u)
         o.set_uniforms(u)
     else:
+        u = counter_stream(2024, cap)
         g.set_seed(2024)
         o.set_seed(2024)
     g.set_log(cap)
@@ -100,12 +102,12 @@ def test_one_sweep_accept_sequence(pkg, hip, oracle, L, U, beta, mu, stream):
     do, po = o.get_log()
     cg, co = g.counters(), o.counters()
     assert len(dg) == len(do) == cap
-    if cg["ties"] == 0 and co["ties"] == 0:
+    if assert_same_decisions(dg, do, po, u):
         assert np.array_equal(dg, do)
         assert ag == ao
         assert np.array_equal(g.get_field(), o.get_field())
         assert cg["uniforms_used"] == co["uniforms_used"]
-        assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+        assert rel_maxdiff(g.greens(), o.greens()) < 1e-10
     # acceptance probabilities agree to rounding
     assert np.abs(pg - po).max() < 1e-8 * max(1.0, np.abs(po).max())
     assert g.state() == o.state() == (1, 1, 1)
@@ -139,11 +141,10 @@ def test_window_widths_and_lattices(pkg, hip, oracle, L, U, beta, mu, lattice, w
     g.sweep(1)
     o.sweep(1)
     dg, _ = g.get_log()
-    do, _ = o.get_log()
-    if g.counters()["ties"] == 0 and o.counters()["ties"] == 0:
-        assert np.array_equal(dg, do)
+    do, po = o.get_log()
+    if assert_same_decisions(dg, do, po, u):
         assert np.array_equal(g.get_field(), o.get_field())
-    assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+        assert rel_maxdiff(g.greens(), o.greens()) < 1e-10
 
 
 def test_u0_closed_form(pkg, hip):
@@ -189,10 +190,10 @@ def test_config_b_sweep_parity(pkg, hip, oracle):
     # stabilisation) measures the same drift as the oracle's, to within rounding order
     assert 0.0 < cg["prop_err_last"] < 1e-6 and 0.0 < co["prop_err_last"] < 1e-6
     assert 0.1 < cg["prop_err_last"] / co["prop_err_last"] < 10.0
-    if cg["ties"] == 0 and co["ties"] == 0:
+    if assert_same_decisions(dg, do, po, u):
         assert mism == 0
         assert np.array_equal(g.get_field(), o.get_field())
-        assert rel_maxdiff(g.greens(), o.greens()) < 1e-9
+        assert rel_maxdiff(g.greens(), o.greens()) < 1e-10
 
 
 @pytest.mark.parametrize("case", ["repulsive", "attractive_mu1"])
