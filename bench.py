@@ -157,6 +157,20 @@ def kernel_bytes(kind, n, flv, win):
     return 0.0
 
 
+def rocprof_avg_us(kernel_tag):
+    """Average duration (us) of a kernel in the committed rocprofv3 --kernel-trace --stats
+    summary of the config-B sweep (profiles/dqmc_sweep_config_b_kernel_stats.md), or None."""
+    path = os.path.join(ROOT, "profiles", "dqmc_sweep_config_b_kernel_stats.md")
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 4 and kernel_tag in cells[0]:
+                return float(cells[3])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def measured_traffic(kernel_tag):
     """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     summary (profiles/dqmc_sweep_config_b_hbm_traffic.md), or None."""
@@ -170,35 +184,39 @@ def measured_traffic(kernel_tag):
     return None
 
 
-def serve_leg(pkg, nreq, rate=20.0, seed=2024):
-    """Requests of uneven shape (square L in {4..12} or honeycomb L in {3..6}, beta in [1, 4],
-    U in [-8, 8], 2..10 sweeps) arriving as a Poisson process of `rate` per second, run through
-    the serving loop (admission by footprint, at most 16 chains at once).  A scaled-down
-    serving workload: the shapes and sweep counts are cut so that the leg takes seconds."""
+def serve_leg(pkg, nreq, rate=20.0, seed=2024, sweep_cap=2):
+    """Config E's request mix through the serving loop: Poisson arrivals at `rate` per second
+    of chains with L uniform in {8..32}, square (N = L^2) or honeycomb (N = 2 L^2) lattice,
+    beta uniform in [5, 20] (M = round(10 beta)), U uniform in [-8, 8], dtau = 0.1, FP64.
+    The drawn sweep count (uniform in {200..2000}) is capped at `sweep_cap` per request so
+    that the leg takes about a minute (a full request runs for up to hours); shapes, beta and
+    U are drawn as in the full mix.  Admission by HBM footprint (64 GiB budget), at most 16
+    chains at once; latency is submit -> result (stack build included)."""
     from montecarlo_jl_amd import _lib as L
     rng = np.random.default_rng(seed)
-    srv = L.Server(memory_budget=16 << 30, max_active=16)
+    srv = L.Server(memory_budget=64 << 30, max_active=16)
     t0 = time.perf_counter()
-    ids, sweeps, t_next = [], 0, 0.0
+    ids, sweeps, t_next, drawn, shapes = [], 0, 0.0, 0, []
     try:
         for k in range(nreq):
             t_next += rng.exponential(1.0 / rate)
             while time.perf_counter() - t0 < t_next:
                 time.sleep(0.001)
-            if rng.random() < 0.5:
-                lat = pkg.SquareLattice(int(rng.choice([4, 6, 8, 10, 12])))
-            else:
-                lat = pkg.Honeycomb(int(rng.integers(3, 7)))
+            Lx = int(rng.integers(8, 33))
+            lat = pkg.SquareLattice(Lx) if rng.random() < 0.5 else pkg.Honeycomb(Lx)
             U = float(rng.uniform(-8.0, 8.0)) or 1.0
-            beta = float(rng.uniform(1.0, 4.0))
-            ns = int(rng.integers(2, 11))
+            beta = float(rng.uniform(5.0, 20.0))
+            ns_full = int(rng.integers(200, 2001))
+            ns = min(ns_full, sweep_cap)
             model = pkg.HubbardModel(l=lat, U=U, mu=0.0)
-            par = pkg.DQMCParameters.create(beta=beta, delta_tau=0.1, safe_mult=10)
+            par = pkg.DQMCParameters.create(beta=beta, delta_tau=0.1, safe_mult=10, warn_round=False)
             field = model.choose_field()(par, model)
             field.rand(1000 + k)
             ids.append(srv.submit(model.hopping_matrix(), field.conf, field.flavors, par.safe_mult,
                                   par.delta_tau, field.alpha, 5000 + k, ns))
             sweeps += ns
+            drawn += ns_full
+            shapes.append(len(lat))
         lats, queued = [], []
         for rid in ids:
             srv.wait(rid)
@@ -214,8 +232,69 @@ def serve_leg(pkg, nreq, rate=20.0, seed=2024):
             "latency_ms": {"p50": float(np.percentile(lats, 50)) * 1e3,
                            "p99": float(np.percentile(lats, 99)) * 1e3},
             "queued_ms_p99": float(np.percentile(queued, 99)) * 1e3,
-            "workload": "square L in {4..12} or honeycomb L in {3..6}, beta U[1,4], U U[-8,8], "
-                        "2..10 sweeps per request, FP64; scaled down from the full serving mix"}
+            "sites": {"min": int(min(shapes)), "max": int(max(shapes)), "mean": float(np.mean(shapes))},
+            "sweep_cap": sweep_cap, "sweeps_drawn": drawn,
+            "workload": "config E mix: square or honeycomb L in {8..32}, beta U[5,20], U U[-8,8], "
+                        f"FP64; sweeps per request capped at {sweep_cap} (drawn from {{200..2000}})"}
+
+
+def config_d_specs(pkg, count, seed=2024):
+    """Config D: L uniform in {6, 8, 10, 12, 16} (square), U uniform in [-8, 8] (reference
+    convention: negative = repulsive, two flavours), beta in {4, 6, 8, 10}, dtau = 0.1, mu = 0,
+    a seeded +-1 field per chain."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        L = int(rng.choice([6, 8, 10, 12, 16]))
+        U = float(rng.uniform(-8.0, 8.0)) or 1.0
+        beta = float(rng.choice([4.0, 6.0, 8.0, 10.0]))
+        model = pkg.HubbardModel(L, 2, U=U)
+        par = pkg.DQMCParameters.create(beta=beta, delta_tau=0.1, safe_mult=10)
+        field = model.choose_field()(par, model)
+        field.rand(100 + k)
+        out.append((model, par, field, 500 + k))
+    return out
+
+
+def batched_leg(pkg, count, sweeps):
+    """Config D: `count` independent FP64 chains swept together by dqmc_batch_sweep (lockstep
+    slice visits on one stream, every launch covering all chains).  Each chain makes `sweeps`
+    sweeps (2 M slice visits each); chains with fewer slices finish earlier.  Per-chain sweep
+    latency: the chain's 2 M visits at the measured time per lockstep step."""
+    from montecarlo_jl_amd import _lib as L
+    specs = config_d_specs(pkg, count)
+    chains = []
+    for model, par, field, seed in specs:
+        c = L.HIPChain(len(model.l), field.flavors, par.slices, par.safe_mult, par.delta_tau, field.alpha)
+        c.set_hopping(model.hopping_matrix())
+        c.set_field(field.conf)
+        c.set_seed(seed)
+        c.build_stack()
+        chains.append(c)
+    try:
+        L.batch_sweep(chains, 1)                       # warm-up
+        t0 = time.perf_counter()
+        L.batch_sweep(chains, sweeps)
+        wall = time.perf_counter() - t0
+    finally:
+        for c in chains:
+            c.close()
+    Ms = np.array([p.slices for _, p, _, _ in specs])
+    ns = np.array([len(m.l) for m, _, _, _ in specs])
+    flv = np.array([f.flavors for _, _, f, _ in specs])
+    step_s = wall / (2 * Ms.max() * sweeps)
+    lat = 2 * Ms * step_s
+    K = -(-Ms // 10)
+    flop = float(np.sum(flv * ns.astype(float) ** 3 * (12 * Ms + 46 * K))) * sweeps
+    return {"workload": "config D: 64 chains, square L in {6,8,10,12,16}, U uniform [-8,8], "
+                        "beta in {4,6,8,10}, dtau=0.1, mu=0, FP64, lockstep slice visits on one stream",
+            "chains": count, "sweeps_per_chain": sweeps, "wall_s": wall,
+            "aggregate_sweeps_per_s": count * sweeps / wall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)) * 1e3, "p99": float(np.percentile(lat, 99)) * 1e3,
+                           "basis": "2 M slice visits x measured time per lockstep step"},
+            "step_us": step_s * 1e6,
+            "dense_tflops": flop / wall / 1e12,
+            "sites": {int(k): int(v) for k, v in zip(*np.unique(ns, return_counts=True))}}
 
 
 def mixed_leg(pkg, sweeps, tol=1e-6):
@@ -232,7 +311,7 @@ def mixed_leg(pkg, sweeps, tol=1e-6):
     for prec in ("fp64", "mixed"):
         ch = make_chain(pkg, model, par, field, prec)
         ch.stabilization_log()
-        lat, logs, failed = [], [], 0
+        lat, logs = [], []
         t0 = time.perf_counter()
         for _ in range(sweeps):
             a = time.perf_counter()
@@ -241,7 +320,6 @@ def mixed_leg(pkg, sweeps, tol=1e-6):
             except L.DQMCError as e:
                 if "(-6)" not in str(e):
                     raise
-                failed += 1
             lat.append(time.perf_counter() - a)
             logs.append(ch.stabilization_log())
         total = time.perf_counter() - t0
@@ -312,13 +390,16 @@ def main():
     ap.add_argument("--skip-mixed", action="store_true")
     ap.add_argument("--mixed-sweeps", type=int, default=3,
                     help="sweeps of the config-C FP64 and mixed-precision chains")
-    ap.add_argument("--batch", type=int, default=8,
-                    help="also time this many config-B chains swept together (0: skip)")
-    ap.add_argument("--serve", type=int, default=24,
-                    help="also run this many requests through the serving loop (0: skip)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="also time this many config-D chains swept together (0: skip)")
+    ap.add_argument("--batch-sweeps", type=int, default=3, help="sweeps per chain of the batched leg")
+    ap.add_argument("--serve", type=int, default=16,
+                    help="also run this many config-E requests through the serving loop (0: skip)")
     ap.add_argument("--cpu-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-procs", type=int, default=16,
                     help="host processes for the all-cores CPU baseline (0: skip)")
+    ap.add_argument("--legs", default="headline,mixed,batched,serving,cpu",
+                    help="comma-separated legs to run (headline always runs unless omitted here)")
     ap.add_argument("--profile-slices", type=int, default=0,
                     help="short run for hardware-counter profiles: build the stack, then this many "
                          "(sweep_spatial + propagate) slice steps, and exit")
@@ -346,6 +427,19 @@ def main():
         ch.synchronize()
         print(json.dumps({"profile_slices": args.profile_slices, "state": ch.state()}))
         ch.close()
+        return
+
+    legs = set(args.legs.split(","))
+    if "headline" not in legs:
+        # a single leg on its own (profiling runs): its JSON object alone
+        out = {}
+        if "batched" in legs:
+            out["batched"] = batched_leg(pkg, args.batch, args.batch_sweeps)
+        if "serving" in legs:
+            out["serving"] = serve_leg(pkg, args.serve)
+        if "mixed" in legs:
+            out["mixed"] = mixed_leg(pkg, args.mixed_sweeps)
+        print(json.dumps(out))
         return
 
     # ---- FP64 headline ------------------------------------------------------------------
@@ -387,6 +481,13 @@ def main():
                     "traffic": measured_traffic("window_kernel"),
                     "avg_launch_us": kernels[dom]["avg_us"], "bytes_per_launch": nbytes,
                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dqmc_sweep_config_b_hbm_traffic.md"}
+        # the HIP events around every launch add their own time: the committed rocprofv3
+        # summary of the same workload gives the kernel's own duration
+        rp = rocprof_avg_us("window_kernel<double, 2, 32>")
+        if rp:
+            roofline["avg_launch_us_rocprof"] = rp
+            roofline["events_over_rocprof"] = kernels[dom]["avg_us"] / rp
+            roofline["frac_rocprof"] = nbytes / (rp * 1e-6) / 1e9 / PEAK["hbm_gbs"]
     else:
         peak = PEAK["fp32_mfma_tflops"] if dom == "gemm_f32" else PEAK["fp64_mfma_tflops"]
         achieved = kernels[dom]["tflops"]
@@ -398,43 +499,23 @@ def main():
 
     # ---- mixed precision at config C, beside the FP64 chain on the same field ---------------
     mixed = None
-    if not args.skip_mixed:
+    if not args.skip_mixed and "mixed" in legs:
         mixed = mixed_leg(pkg, args.mixed_sweeps)
 
-    # ---- batched form: several independent chains, one stream and host thread each ---------
+    # ---- batched form: config D, 64 chains in lockstep, one launch per kernel kind ---------
     batched = None
-    if args.batch > 1:
-        from montecarlo_jl_amd import _lib as L
-        chains = []
-        for k in range(args.batch):
-            f = model.choose_field()(par, model)
-            f.rand(1 + k)
-            c = L.HIPChain(n, flv, M, par.safe_mult, par.delta_tau, f.alpha)
-            c.set_hopping(model.hopping_matrix())
-            c.set_field(f.conf)
-            c.set_seed(12345 + k)
-            c.build_stack()
-            chains.append(c)
-        L.batch_sweep(chains, 1)
-        bsteps = max(2, args.steps // 4)
-        t0 = time.perf_counter()
-        L.batch_sweep(chains, bsteps)
-        bt = time.perf_counter() - t0
-        batched = {"chains": args.batch, "sweeps_per_chain": bsteps,
-                   "aggregate_sweeps_per_s": args.batch * bsteps / bt,
-                   "per_chain_sweeps_per_s": bsteps / bt,
-                   "speedup_vs_one_chain": (args.batch * bsteps / bt) / sps}
-        for c in chains:
-            c.close()
+    if args.batch > 1 and "batched" in legs:
+        batched = batched_leg(pkg, args.batch, args.batch_sweeps)
+        batched["aggregate_vs_one_config_b_chain"] = batched["aggregate_sweeps_per_s"] / sps
 
     # ---- serving form: Poisson arrivals through the request loop -----------------------------
     serving = None
-    if args.serve > 0:
+    if args.serve > 0 and "serving" in legs:
         serving = serve_leg(pkg, args.serve)
 
     # ---- CPU baseline --------------------------------------------------------------------
     cpu = None
-    if not args.skip_cpu:
+    if not args.skip_cpu and "cpu" in legs:
         cps, cdt = cpu_baseline(pkg, model, par, field, args.cpu_sweeps)
         cpu = {"value": cps, "unit": "sweeps/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_sweeps} sweeps of the same chain (N=256, M=100, 2 flavours) "

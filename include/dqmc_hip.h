@@ -134,11 +134,16 @@ int dqmc_sweep_spatial(dqmc_chain_t h, int64_t* accepted);
 /* nsweeps x local_sweep (local_updates.jl:7-14): 2M x (sweep_spatial; propagate).      */
 int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted);
 
-/* Batched form: `count` distinct chains sweep `nsweeps` times each, concurrently (one host
- * thread and one stream per chain), each exactly as dqmc_sweep would alone.  accepted[i]
- * (optional) receives chain i's accepted flips.  No reference counterpart: the reference
- * runs one chain per Julia process (run!, DQMC.jl:144-394); this is the many-chain form of
- * update(::LocalSweep) (local_updates.jl:82) for independent chains. */
+/* Batched form: `count` distinct chains sweep `nsweeps` times each, each exactly as
+ * dqmc_sweep would alone (bit for bit).  accepted[i] (optional) receives chain i's accepted
+ * flips.  FP64 chains on one device run in lockstep slice visits on one stream: every launch
+ * covers all chains (window kernels grouped by flavours and window width, the propagate
+ * kernels of chains at the same kind of step together), with per-chain item tables in device
+ * memory; chains with fewer slices finish their sweeps earlier.  Other mixes (mixed precision,
+ * several devices, kernel timing on) run one host thread and stream per chain.  No reference
+ * counterpart: the reference runs one chain per Julia process (run!, DQMC.jl:252-394); this
+ * is the many-chain form of update(::LocalSweep) (local_updates.jl:82) for independent
+ * chains. */
 int dqmc_batch_sweep(dqmc_chain_t* chains, int count, int nsweeps, int64_t* accepted);
 
 /* Serving form: a request loop over independent chains with continuous batching.  No
@@ -201,6 +206,16 @@ int dqmc_set_greens(dqmc_chain_t h, const double* G, int ld);
 /* calculate_greens(mc, slice, output) from scratch (stack.jl:525-583); does not touch
  * the chain's current G.                                                               */
 int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* G, int ld);
+
+/* global_update(mc, model, field) with `proposal` (n x slices, +-1) as the proposed conf
+ * (global_updates.jl:191-207; propose_global_from_conf :136-169, inv_det :64-121,
+ * calculate_inv_greens_udt :25-47, accept_global! :171-188): p = |exp(-dE) detratio| from
+ * the D vectors of the current G and of the proposal's G from scratch; accepted iff p > 1 or
+ * u < p with u the chain's next uniform (consumed only when p <= 1); an accepted proposal
+ * becomes the field and the stack is rebuilt (reverse_build_stack + propagate), a rejected
+ * one leaves field and G as they were.  Needs current_slice == 1 and direction == 1 right
+ * after a sweep or a stack build.  *accepted and *p (either may be NULL) get the outcome. */
+int dqmc_global_update(dqmc_chain_t h, const int8_t* proposal, int ld, int* accepted, double* p);
 
 int dqmc_get_state(dqmc_chain_t h, int* current_slice, int* direction, int* current_range);
 int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* stats);

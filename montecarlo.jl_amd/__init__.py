@@ -9,11 +9,17 @@ from .model import (HubbardModel, AttractiveHubbardModel, RepulsiveHubbardModel,
                     HubbardModelAttractive, HubbardModelRepulsive)
 from .parameters import DQMCParameters, generate_chunks
 from .fields import DensityHirschField, MagneticHirschField, hirsch_alpha
-from .dqmc import DQMC, run, ExitCode, LocalSweep, MagnitudeStats, DQMCAnalysis
+from .dqmc import DQMC, run, ExitCode, MagnitudeStats, DQMCAnalysis
+from .updates import (LocalSweep, GlobalFlip, GlobalShuffle, SpatialShuffle, TemporalShuffle, Denoise,
+                      DenoiseFlip, StaggeredDenoise, SimpleScheduler, AcceptanceStatistics)
+from .fileio import ConfigRecorder, Discarder, save, load, resume
 from . import measurements
 
 __all__ = ["Chain", "SquareLattice", "CubicLattice", "Honeycomb", "TriangularLattice", "Lattice",
            "HubbardModel", "AttractiveHubbardModel", "RepulsiveHubbardModel",
            "HubbardModelAttractive", "HubbardModelRepulsive", "DQMCParameters", "generate_chunks",
            "DensityHirschField", "MagneticHirschField", "hirsch_alpha", "DQMC", "run", "ExitCode",
-           "LocalSweep", "MagnitudeStats", "DQMCAnalysis", "measurements"]
+           "LocalSweep", "GlobalFlip", "GlobalShuffle", "SpatialShuffle", "TemporalShuffle", "Denoise",
+           "DenoiseFlip", "StaggeredDenoise", "SimpleScheduler", "AcceptanceStatistics",
+           "ConfigRecorder", "Discarder", "save", "load", "resume", "MagnitudeStats", "DQMCAnalysis",
+           "measurements"]

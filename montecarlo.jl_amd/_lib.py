@@ -55,7 +55,8 @@ EXPORTED = [
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
-    "dqmc_calculate_greens", "dqmc_get_state", "dqmc_get_stats", "dqmc_set_wait_limit",
+    "dqmc_calculate_greens", "dqmc_global_update", "dqmc_get_state", "dqmc_get_stats",
+    "dqmc_set_wait_limit",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
     "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
@@ -105,6 +106,7 @@ def load():
         "dqmc_get_greens": [vp, _dp, i, i],
         "dqmc_set_greens": [vp, _dp, i],
         "dqmc_calculate_greens": [vp, i, _dp, i],
+        "dqmc_global_update": [vp, _i8, i, C.POINTER(i), C.POINTER(d)],
         "dqmc_get_state": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
         "dqmc_get_stats": [vp, C.POINTER(Stats)],
         "dqmc_set_wait_limit": [vp, i64],
@@ -380,6 +382,14 @@ class HIPChain:
         G = np.zeros((self.n, self.n * self.flv), order="F")
         self._c(load().dqmc_calculate_greens(self._h, slice_, G, self.n), "dqmc_calculate_greens")
         return self._blocks(G)
+
+    def global_update(self, proposal):
+        """global_update with `proposal` as the new conf (dqmc_global_update): (accepted, p)."""
+        c = _f(proposal, np.int8)
+        acc, p = C.c_int(), C.c_double()
+        self._c(load().dqmc_global_update(self._h, c, c.shape[0], C.byref(acc), C.byref(p)),
+                "dqmc_global_update")
+        return bool(acc.value), p.value
 
     def state(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -59,6 +59,10 @@ struct GemmBatch {
   int count;
   int transA, transB;
   double alpha, beta;
+  // many-chain launches: `count` items in device memory (it[] unused); max_m / max_n size
+  // the grid (every item carries its own M, N, K)
+  const GemmItem<T>* items = nullptr;
+  int max_m = 0, max_n = 0;
 };
 
 // Launchers (gemm.hip)
@@ -75,6 +79,10 @@ struct UdtItem {
   double* tau;      // n scratch
   double* tfac;     // (n + 16) * 16 scratch (compact-WY T factors, 16 x 16 per panel start)
   double* swap;     // 32 * n scratch (column moves)
+  unsigned long long* err;   // device error counter (may be null): +1 per broken pivot selection
+  // many-chain launches (item table in device memory): the item's own size and, for the
+  // panel and reflector launches, its panel (start, width, last panel)
+  int n, p0, w, last;
 };
 constexpr int kMaxUdtItems = 4;
 struct UdtBatch {
@@ -82,7 +90,10 @@ struct UdtBatch {
   int count;
   int n;
   int apply_pivot;
-  unsigned long long* err;   // device error counter (may be null): +1 per broken pivot selection
+  // many-chain launches: `count` items in device memory, each with its own n (<= max_n)
+  // and panel; it[] and n unused
+  const UdtItem* items = nullptr;
+  int max_n = 0;
 };
 hipError_t launch_udt(const UdtBatch& b, hipStream_t s);
 // The steps of launch_udt, for callers that time them separately: panel starts and widths,
@@ -90,11 +101,21 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s);
 // update of A from column c0 with residual norms; mode 1: Q accumulation into U from
 // column c0), D/T output.
 void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths);
+// The same plan into caller arrays of `cap` entries (no allocation); returns the panel
+// count, or -1 if it exceeds cap.
+int udt_plan_fixed(int n, int* starts, int* widths, int cap);
 hipError_t udt_check(const UdtBatch& b);
 hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s);
 hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s);
 hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, hipStream_t s);
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s);
+// Item-table forms (b.items set): every item's panel (p0, w, last) is in its table entry;
+// the panel kernel's rows-per-thread class R (1, 2, 4, 8: rows <= 256 R) is the same for all
+// items of one launch.  max_w bounds the items' panel widths.
+int udt_rows_class(int rows);
+int udt_width_for(int rows);
+hipError_t launch_udt_panel_tab(const UdtBatch& b, int R, hipStream_t s);
+hipError_t launch_udt_reflect_tab(const UdtBatch& b, int max_grid, hipStream_t s);
 // Q accumulation into U (U preset to I) for every panel, in one launch
 
 
@@ -104,12 +125,16 @@ struct RdivpItem {
   const double* T;
   double* O;
   const int* pivot;
+  int n;            // item-table launches: the item's size
 };
 struct RdivpBatch {
   RdivpItem it[kMaxUdtItems];
   int count;
   int n;
+  const RdivpItem* items = nullptr;   // many-chain launches: device table, own n <= max_n
+  int max_n = 0;
 };
+constexpr int kRdivpStripMaxN = 1024;  // one wave per 16-row strip up to this size
 hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s);
 
 // Elementwise helpers (util.hip)
@@ -136,7 +161,28 @@ hipError_t launch_norm1(const double* A, int n, double* out, hipStream_t s);
 hipError_t launch_scale_cols(double* out, const double* in, const double* d, int n, int nmat,
                              hipStream_t s);
 
+// Many-chain elementwise launches: one item per (chain, buffer), in device memory.
+enum ElemOp { kElemCopy = 0, kElemIdentity = 1, kElemFill = 2 };
+struct ElemItem {
+  int op;                 // ElemOp
+  int n;                  // identity: matrix size (count = n * n * nmat)
+  double* dst;
+  const double* src;      // copy
+  double v;               // fill value
+  int64_t count;          // elements written
+};
+hipError_t launch_elem_multi(const ElemItem* items, int count, int64_t max_count, hipStream_t s);
+// out[k] = max |a_k - b_k| over count_k elements (out[k] preset to +0.0)
+struct MaxdiffItem {
+  const double* a;
+  const double* b;
+  int64_t count;
+  double* out;
+};
+hipError_t launch_maxabsdiff_multi(const MaxdiffItem* items, int count, int64_t max_count, hipStream_t s);
+
 // One-time kernel attribute setup (dynamic LDS above 64 KB).
+hipError_t kernels_init_once();   // all of the below, once per process (dqmc.hip)
 hipError_t qr_kernels_init();
 hipError_t gemm_kernels_init();
 hipError_t local_kernels_init();
@@ -186,6 +232,12 @@ struct WindowArgs {
 };
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
+// Many-chain window launch: `count` chains' WindowArgs in device memory (same flv and W for
+// all).  Workgroup (0, c) runs chain c's window when its do_window is set; workgroups
+// (1 + t, c) apply chain c's previous window to tile t when its Yp is set.  max_n sizes the
+// grid; every item carries its own n.
+hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int flv, int W, int max_n,
+                                   hipStream_t s);
 int window_size(int n, int flv);
 
 }  // namespace dqmc

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -19,6 +20,7 @@
 
 #include "../../include/dqmc_hip.h"
 #include "common.h"
+#include "ensemble.h"
 
 using namespace dqmc;
 
@@ -34,6 +36,23 @@ void init_kernels() {
   g_init_status = e;
 }
 }  // namespace
+
+hipError_t dqmc::kernels_init_once() {
+  std::call_once(g_init_once, init_kernels);
+  return g_init_status;
+}
+
+// One kernel of a chain's propagate, recorded instead of launched when the chain runs in
+// a many-chain step: the step's launches then cover every chain whose recorded sequence
+// has the same kinds (each chain's items in its own order, so its results are those of
+// its own launches).
+enum OpKind { kOpGemm = 0, kOpUdt = 1, kOpRdivp = 2, kOpElem = 3, kOpMaxdiff = 4 };
+struct RecOp {
+  int kind;
+  int key;           // gemm: 2 transA + transB; udt: apply_pivot
+  double alpha, beta;
+  int first, count;  // items in the chain's per-kind record
+};
 
 struct dqmc_chain_s {
   int dev = 0;
@@ -90,6 +109,20 @@ struct dqmc_chain_s {
   double perr_max = 0.0, perr_last = 0.0;
   std::vector<double> stablog;
   std::string err;
+
+  // many-chain steps: record mode collects the chain's kernels (RecOp + items; window
+  // launches) instead of launching them
+  bool rec = false;
+  hipStream_t own_st = nullptr;      // the chain's stream while it runs in an ensemble
+  std::vector<RecOp> ops;
+  std::vector<GemmItem<double>> rgemm;
+  std::vector<UdtItem> rudt;
+  std::vector<RdivpItem> rrdivp;
+  std::vector<ElemItem> relem;
+  std::vector<MaxdiffItem> rmax;
+  std::vector<WindowArgs<double>> rwin;
+  long long steps = 0;               // slice visits (sweep_spatial + propagate) in ensembles
+  bool dg_valid = false;             // Dr holds the D of the last calculate_greens of G itself
 
   // optional per-kernel timing with HIP events on the chain's stream
   bool prof = false;
@@ -261,10 +294,25 @@ void collect_times(Chain* h) {
   h->pending.clear();
 }
 
+int record(Chain* h, int kind, int key, int first, int count, double alpha = 1.0, double beta = 0.0) {
+  h->ops.push_back(RecOp{kind, key, alpha, beta, first, count});
+  return DQMC_OK;
+}
+
 int run(Chain* h, const GemmBatch<double>& b) {
+  if (h->rec) {
+    const int first = (int)h->rgemm.size();
+    for (int i = 0; i < b.count; i++) h->rgemm.push_back(b.it[i]);
+    return record(h, kOpGemm, 2 * b.transA + b.transB, first, b.count, b.alpha, b.beta);
+  }
   Timed t(h, DQMC_KERNEL_GEMM_F64);
   HIPCHK(h, launch_gemm_f64(b, h->st));
   return DQMC_OK;
+}
+
+int elem(Chain* h, const ElemItem& e) {
+  h->relem.push_back(e);
+  return record(h, kOpElem, 0, (int)h->relem.size() - 1, 1);
 }
 int run(Chain* h, const GemmBatch<float>& b) {
   Timed t(h, DQMC_KERNEL_GEMM_F32);
@@ -273,16 +321,22 @@ int run(Chain* h, const GemmBatch<float>& b) {
 }
 
 int copyd(Chain* h, double* dst, const double* src, size_t count) {
+  if (h->rec) {
+    if (count == 0 || dst == src) return DQMC_OK;
+    return elem(h, ElemItem{kElemCopy, 0, dst, src, 0.0, (int64_t)count});
+  }
   HIPCHK(h, launch_copy(dst, src, (int64_t)count, h->st));
   return DQMC_OK;
 }
 
 int identity(Chain* h, double* A, int nmat) {
+  if (h->rec) return elem(h, ElemItem{kElemIdentity, h->n, A, nullptr, 0.0, (int64_t)h->nn * nmat});
   HIPCHK(h, launch_set_identity(A, h->n, nmat, h->st));
   return DQMC_OK;
 }
 
 int ones(Chain* h, double* d, size_t count) {
+  if (h->rec) return elem(h, ElemItem{kElemFill, 0, d, nullptr, 1.0, (int64_t)count});
   HIPCHK(h, launch_fill(d, 1.0, (int64_t)count, h->st));
   return DQMC_OK;
 }
@@ -368,8 +422,9 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
   u.count = h->flv;
   u.n = h->n;
   u.apply_pivot = apply_pivot;
-  u.err = h->ctr ? reinterpret_cast<unsigned long long*>(&h->ctr->device_errors) : nullptr;
   for (int b = 0; b < h->flv; b++) {
+    u.it[b].err = h->ctr ? reinterpret_cast<unsigned long long*>(&h->ctr->device_errors) : nullptr;
+    u.it[b].n = h->n; u.it[b].p0 = u.it[b].w = u.it[b].last = 0;
     u.it[b].A = A[b]; u.it[b].U = U[b]; u.it[b].D = D[b]; u.it[b].T = T[b];
     u.it[b].pivot = h->piv + (size_t)b * h->n;
     u.it[b].tau = h->tau + (size_t)b * h->n;
@@ -377,6 +432,11 @@ int udt(Chain* h, double* const A[], double* const U[], double* const D[], doubl
     u.it[b].swap = h->swp + (size_t)b * 32 * h->n;
   }
   HIPCHK(h, udt_check(u));
+  if (h->rec) {
+    const int first = (int)h->rudt.size();
+    for (int b = 0; b < u.count; b++) h->rudt.push_back(u.it[b]);
+    return record(h, kOpUdt, apply_pivot, first, u.count);
+  }
   std::vector<int> st, wd;
   udt_plan(h->n, st, wd);
   const int np = (int)st.size();
@@ -408,6 +468,12 @@ int rdivp(Chain* h, double* const A[], double* const T[]) {
     r.it[b].A = A[b]; r.it[b].T = T[b];
     r.it[b].O = blk(h->O, h, b);
     r.it[b].pivot = h->piv + (size_t)b * h->n;
+    r.it[b].n = h->n;
+  }
+  if (h->rec) {
+    const int first = (int)h->rrdivp.size();
+    for (int b = 0; b < r.count; b++) h->rrdivp.push_back(r.it[b]);
+    return record(h, kOpRdivp, 0, first, r.count);
   }
   Timed t(h, DQMC_KERNEL_RDIVP);
   HIPCHK(h, launch_rdivp(r, h->st));
@@ -464,6 +530,8 @@ int add_slice_sequence_right(Chain* h, int idx) {
 // calculate_greens_AVX!(Ul, Dl, Tl, Ur, Dr, Tr, G), stack.jl:442-496
 int calculate_greens(Chain* h, double* out) {
   const int n = h->n;
+  // the final D stays in Dr; it belongs to the chain's own G only when that is the output
+  h->dg_valid = out == h->G;
   Blocks G = B(h, out), Ul = B(h, h->Ul), Ur = B(h, h->Ur), Tl = B(h, h->Tl), Tr = B(h, h->Tr);
   Blocks Dl = Bd(h, h->Dl), Dr = Bd(h, h->Dr);
   {  // G = Dl (Tl Tr') Dr
@@ -543,6 +611,14 @@ int record_error(Chain* h, const double* a, const float* a32, const double* b) {
     HIPCHK(h, hipStreamSynchronize(h->st));
     for (double x : v) absorb_error(h, x);
     h->nerr = 0;
+  }
+  if (h->rec) {
+    // the slot starts at +0.0 (a fill), then the maximum is combined into it
+    double* out = h->errbuf + h->nerr;
+    h->nerr++;
+    CHK(elem(h, ElemItem{kElemFill, 0, out, nullptr, 0.0, 1}));
+    h->rmax.push_back(MaxdiffItem{a, b, (int64_t)(h->flv * h->nn), out});
+    return record(h, kOpMaxdiff, 0, (int)h->rmax.size() - 1, 1);
   }
   HIPCHK(h, launch_maxabsdiff(a, a32, b, (int64_t)h->flv * h->nn, h->errbuf + h->nerr, h->st));
   h->nerr++;
@@ -691,6 +767,12 @@ int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
   a.wait_limit = h->wait_limit;
   int slot = 0;
   auto launch = [&](int kind) -> int {
+    if constexpr (sizeof(GT) == 8) {
+      if (h->rec) {
+        h->rwin.push_back(a);
+        return DQMC_OK;
+      }
+    }
     Timed t(h, kind);
     hipError_t e;
     if constexpr (sizeof(GT) == 8) e = launch_window_f64(a, h->st);
@@ -797,7 +879,456 @@ int64_t footprint(const dqmc_config_t* c, int* Kout) {
   return b;
 }
 
+int read_accepted(Chain* h, long long* acc) {
+  ChainCounters c;
+  HIPCHK(h, hipMemcpy(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost));
+  *acc = c.accepted;
+  return DQMC_OK;
+}
+
+// calculate_greens(mc, slice, output) from scratch (stack.jl:525-583): the chain of slice
+// products with a UDT every safe_mult slices (k % safe_mult == 0) on both sides of `slice`,
+// then calculate_greens into `out`.  Uses the stack's temporaries, not its slots.
+int greens_from_scratch(Chain* h, int slice, double* out) {
+  const int sm = h->sm, M = h->M;
+  const size_t m = (size_t)h->flv * h->nn;
+  double* ca = h->cur;
+  double* cb = h->tmp2;
+  auto chain_step = [&](double* Ux, double* Dx, double* Tx, bool final) -> int {
+    HIPCHK(h, launch_scale_cols(h->tmp1, ca, Dx, h->n, h->flv, h->st));
+    CHK(udt(h, B(h, h->tmp1).get(), B(h, final ? Ux : ca).get(), Bd(h, Dx).get(), B(h, h->O).get(), 1));
+    CHK(mul_blocks(h, B(h, h->Gx), B(h, h->O), B(h, Tx)));
+    return copyd(h, Tx, h->Gx, m);
+  };
+  CHK(identity(h, ca, h->flv));
+  CHK(udt_identity(h, h->Ur, h->Dr, h->Tr));
+  if (slice + 1 <= M) {
+    for (int k = M; k >= slice + 1; k--) {
+      CHK(slice_dagger_left(h, k, B(h, ca).get(), B(h, cb).get(), nullptr));
+      std::swap(ca, cb);
+      if (k % sm == 0) CHK(chain_step(nullptr, h->Dr, h->Tr, false));
+    }
+    CHK(chain_step(h->Ur, h->Dr, h->Tr, true));
+  }
+  CHK(identity(h, ca, h->flv));
+  CHK(udt_identity(h, h->Ul, h->Dl, h->Tl));
+  if (slice >= 1) {
+    for (int k = 1; k <= slice; k++) {
+      CHK(slice_left(h, k, B(h, ca).get(), B(h, cb).get(), nullptr));
+      std::swap(ca, cb);
+      if (k % sm == 0) CHK(chain_step(nullptr, h->Dl, h->Tl, false));
+    }
+    CHK(chain_step(h->Ul, h->Dl, h->Tl, true));
+  }
+  return calculate_greens(h, out);
+}
+
+// The counter stream of the window kernels, u_k = SplitMix64(seed, k) >> 11 * 2^-53.
+double counter_uniform_host(unsigned long long seed, unsigned long long k) {
+  unsigned long long z = seed + (k + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+// ---- many-chain steps -------------------------------------------------------------------
+// One step is one slice visit of every chain: sweep_spatial + propagate, the body of
+// local_sweep (local_updates.jl:7-14).  Each chain's kernels are recorded, then launched
+// together: window launch k covers the k-th window launch of every chain (grouped by
+// flavours and window width), and the propagate kernels of the chains whose recorded
+// sequences have the same kinds share one launch per position (a chain's items run in its
+// own order, so every chain computes exactly what its own launches would).  A step's item
+// tables are packed into pinned host memory and reach the device in one copy; a ring of
+// slots lets the host pack step t + 1 while the device runs step t.
+struct Executor {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  static constexpr int kSlots = 4;
+  char* host[kSlots] = {};
+  char* devb[kSlots] = {};
+  size_t cap[kSlots] = {};
+  hipEvent_t ev[kSlots] = {};
+  bool pending[kSlots] = {};
+  int slot = 0;
+  std::string err;
+  std::vector<char> stage;
+  struct Launch {
+    int kind;          // 0 window, 1 gemm, 2 udt norms, 3 udt panel, 4 udt reflect, 5 udt finish,
+                       // 6 rdivp, 7 elem, 8 maxdiff
+    size_t off;
+    int count;
+    int a, b, c;       // window: flv, W, max_n; gemm: transA, transB, -; udt: max_n, R|apply, grid
+    int64_t m1, m2;    // gemm max_m, max_n; elem / maxdiff max count
+    double alpha, beta;
+  };
+  std::vector<Launch> launches;
+
+  ~Executor() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (int k = 0; k < kSlots; k++) {
+      if (host[k]) (void)hipHostFree(host[k]);
+      if (devb[k]) (void)hipFree(devb[k]);
+      if (ev[k]) (void)hipEventDestroy(ev[k]);
+    }
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  template <typename T>
+  size_t put(const T* p, size_t count) {
+    size_t off = (stage.size() + 255) & ~(size_t)255;
+    stage.resize(off + sizeof(T) * count);
+    std::memcpy(stage.data() + off, p, sizeof(T) * count);
+    return off;
+  }
+};
+
+#define XCHK(x, expr)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      (x)->err = std::string(#expr) + ": " + hipGetErrorString(e_);         \
+      return DQMC_E_HIP;                                                    \
+    }                                                                       \
+  } while (0)
+
+int executor_init(Executor* x, int dev) {
+  x->dev = dev;
+  XCHK(x, hipSetDevice(dev));
+  XCHK(x, hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking));
+  for (int k = 0; k < Executor::kSlots; k++) XCHK(x, hipEventCreateWithFlags(&x->ev[k], hipEventDisableTiming));
+  return DQMC_OK;
+}
+
+// A chain joins: its queued work finishes, then its kernels go to the executor's stream.
+int executor_attach(Executor* x, Chain* h) {
+  if (h->own_st) { x->err = "chain already runs in an ensemble"; return DQMC_E_STATE; }
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  h->own_st = h->st;
+  h->st = x->st;
+  return DQMC_OK;
+}
+
+// A chain leaves: everything queued for it has run, and it has its own stream again.
+int executor_detach(Executor* x, Chain* h) {
+  if (!h->own_st) return DQMC_OK;
+  XCHK(x, hipStreamSynchronize(x->st));
+  h->st = h->own_st;
+  h->own_st = nullptr;
+  return DQMC_OK;
+}
+
+// Item tables of one UDT op for all chains of a group (each item with its own n): norms,
+// per panel index the panel launches (one per rows-per-thread class) and the reflector
+// launch, then D and T.
+void pack_udt(Executor* x, std::vector<UdtItem>& items, int apply_pivot) {
+  int max_n = 0;
+  for (auto& it : items) max_n = std::max(max_n, it.n);
+  x->launches.push_back({2, x->put(items.data(), items.size()), (int)items.size(), max_n, 0, 0, 0, 0, 0, 0});
+  std::vector<std::vector<int>> st(items.size()), wd(items.size());
+  int np = 0;
+  for (size_t i = 0; i < items.size(); i++) {
+    udt_plan(items[i].n, st[i], wd[i]);
+    np = std::max(np, (int)st[i].size());
+  }
+  std::vector<UdtItem> tab;
+  for (int p = 0; p < np; p++) {
+    tab.clear();
+    for (int R : {1, 2, 4, 8})
+      for (size_t i = 0; i < items.size(); i++) {
+        if (p >= (int)st[i].size()) continue;
+        UdtItem u = items[i];
+        u.p0 = st[i][p];
+        u.w = wd[i][p];
+        u.last = p == (int)st[i].size() - 1;
+        if (udt_rows_class(u.n - u.p0) == R) tab.push_back(u);
+      }
+    const size_t off = x->put(tab.data(), tab.size());
+    int grid = 0, i0 = 0;
+    while (i0 < (int)tab.size()) {
+      const int R = udt_rows_class(tab[i0].n - tab[i0].p0);
+      int i1 = i0, mn = 0;
+      while (i1 < (int)tab.size() && udt_rows_class(tab[i1].n - tab[i1].p0) == R) mn = std::max(mn, tab[i1++].n);
+      x->launches.push_back({3, off + sizeof(UdtItem) * i0, i1 - i0, mn, R, 0, 0, 0, 0, 0});
+      i0 = i1;
+    }
+    for (auto& u : tab) {
+      const int c0 = u.p0 + u.w;
+      grid = std::max(grid, (c0 < u.n ? (u.n - c0 + 15) / 16 : 0) + (u.n + 15) / 16);
+    }
+    x->launches.push_back({4, off, (int)tab.size(), 0, 0, grid, 0, 0, 0, 0});
+  }
+  x->launches.push_back({5, x->put(items.data(), items.size()), (int)items.size(), max_n, apply_pivot, 0, 0, 0, 0, 0});
+}
+
+int executor_launch(Executor* x, const Executor::Launch& L, const char* base) {
+  const void* p = base + L.off;
+  switch (L.kind) {
+    case 0:
+      XCHK(x, launch_window_multi_f64(static_cast<const WindowArgs<double>*>(p), L.count, L.a, L.b, L.c, x->st));
+      break;
+    case 1: {
+      GemmBatch<double> g;
+      g.count = L.count; g.transA = L.a; g.transB = L.b; g.alpha = L.alpha; g.beta = L.beta;
+      g.items = static_cast<const GemmItem<double>*>(p);
+      g.max_m = (int)L.m1; g.max_n = (int)L.m2;
+      XCHK(x, launch_gemm_f64(g, x->st));
+      break;
+    }
+    case 2: case 3: case 4: case 5: {
+      UdtBatch u;
+      u.count = L.count; u.n = 0; u.apply_pivot = L.kind == 5 ? L.b : 0;
+      u.items = static_cast<const UdtItem*>(p);
+      u.max_n = L.a;
+      if (L.kind == 2) XCHK(x, launch_udt_norms(u, x->st));
+      else if (L.kind == 3) XCHK(x, launch_udt_panel_tab(u, L.b, x->st));
+      else if (L.kind == 4) XCHK(x, launch_udt_reflect_tab(u, L.c, x->st));
+      else XCHK(x, launch_udt_finish(u, x->st));
+      break;
+    }
+    case 6: {
+      RdivpBatch r;
+      r.count = L.count; r.n = 0;
+      r.items = static_cast<const RdivpItem*>(p);
+      r.max_n = L.a;
+      XCHK(x, launch_rdivp(r, x->st));
+      break;
+    }
+    case 7: XCHK(x, launch_elem_multi(static_cast<const ElemItem*>(p), L.count, L.m1, x->st)); break;
+    case 8: XCHK(x, launch_maxabsdiff_multi(static_cast<const MaxdiffItem*>(p), L.count, L.m1, x->st)); break;
+    default: x->err = "unknown launch kind"; return DQMC_E_STATE;
+  }
+  return DQMC_OK;
+}
+
+// One slice visit of every chain in `cs` (all attached, FP64, stack built).
+int executor_step(Executor* x, Chain* const* cs, int nc) {
+  x->stage.clear();
+  x->launches.clear();
+  for (int i = 0; i < nc; i++) {
+    Chain* h = cs[i];
+    h->ops.clear(); h->rgemm.clear(); h->rudt.clear(); h->rrdivp.clear(); h->relem.clear(); h->rmax.clear();
+    h->rwin.clear();
+    h->rec = true;
+    int r = sweep_spatial(h);
+    if (r == DQMC_OK) r = propagate(h);
+    h->rec = false;
+    if (r != DQMC_OK) { x->err = h->err; return r; }
+  }
+  // window launches: position k of every chain, grouped by (flavours, window width)
+  size_t kmax = 0;
+  for (int i = 0; i < nc; i++) kmax = std::max(kmax, cs[i]->rwin.size());
+  std::vector<WindowArgs<double>> wtab;
+  for (size_t k = 0; k < kmax; k++)
+    for (int flv = 1; flv <= 2; flv++)
+      for (int W : {16, 32, 64}) {
+        wtab.clear();
+        int mn = 0;
+        for (int i = 0; i < nc; i++)
+          if (cs[i]->rwin.size() > k && cs[i]->flv == flv && cs[i]->win == W) {
+            wtab.push_back(cs[i]->rwin[k]);
+            mn = std::max(mn, cs[i]->n);
+          }
+        if (!wtab.empty())
+          x->launches.push_back({0, x->put(wtab.data(), wtab.size()), (int)wtab.size(), flv, W, mn, 0, 0, 0, 0});
+      }
+  // propagate: chains grouped by the kinds of their recorded sequences
+  std::map<std::vector<int64_t>, std::vector<int>> groups;
+  for (int i = 0; i < nc; i++) {
+    std::vector<int64_t> sig;
+    sig.reserve(cs[i]->ops.size() * 3);
+    for (const RecOp& o : cs[i]->ops) {
+      sig.push_back(o.kind * 8 + o.key);
+      int64_t ab[2];
+      std::memcpy(ab, &o.alpha, 8);
+      std::memcpy(ab + 1, &o.beta, 8);
+      sig.push_back(ab[0]);
+      sig.push_back(ab[1]);
+    }
+    groups[sig].push_back(i);
+  }
+  std::vector<GemmItem<double>> gt;
+  std::vector<UdtItem> ut;
+  std::vector<RdivpItem> rt, rt2;
+  std::vector<ElemItem> et;
+  std::vector<MaxdiffItem> mt;
+  for (auto& kv : groups) {
+    const std::vector<int>& mem = kv.second;
+    const size_t nops = cs[mem[0]]->ops.size();
+    for (size_t q = 0; q < nops; q++) {
+      const RecOp& o0 = cs[mem[0]]->ops[q];
+      switch (o0.kind) {
+        case kOpGemm: {
+          gt.clear();
+          int64_t mm = 0, mnn = 0;
+          for (int i : mem) {
+            const RecOp& o = cs[i]->ops[q];
+            for (int k = 0; k < o.count; k++) {
+              const GemmItem<double>& g = cs[i]->rgemm[o.first + k];
+              gt.push_back(g);
+              mm = std::max<int64_t>(mm, g.M);
+              mnn = std::max<int64_t>(mnn, g.N);
+            }
+          }
+          x->launches.push_back({1, x->put(gt.data(), gt.size()), (int)gt.size(), o0.key >> 1, o0.key & 1, 0,
+                                 mm, mnn, o0.alpha, o0.beta});
+          break;
+        }
+        case kOpUdt: {
+          ut.clear();
+          for (int i : mem) {
+            const RecOp& o = cs[i]->ops[q];
+            for (int k = 0; k < o.count; k++) ut.push_back(cs[i]->rudt[o.first + k]);
+          }
+          pack_udt(x, ut, o0.key);
+          break;
+        }
+        case kOpRdivp: {
+          rt.clear(); rt2.clear();
+          int m1 = 0, m2 = 0;
+          for (int i : mem) {
+            const RecOp& o = cs[i]->ops[q];
+            for (int k = 0; k < o.count; k++) {
+              const RdivpItem& r = cs[i]->rrdivp[o.first + k];
+              if (r.n <= kRdivpStripMaxN) { rt.push_back(r); m1 = std::max(m1, r.n); }
+              else { rt2.push_back(r); m2 = std::max(m2, r.n); }
+            }
+          }
+          if (!rt.empty()) x->launches.push_back({6, x->put(rt.data(), rt.size()), (int)rt.size(), m1, 0, 0, 0, 0, 0, 0});
+          if (!rt2.empty()) x->launches.push_back({6, x->put(rt2.data(), rt2.size()), (int)rt2.size(), m2, 0, 0, 0, 0, 0, 0});
+          break;
+        }
+        case kOpElem: {
+          et.clear();
+          int64_t mc = 0;
+          for (int i : mem) {
+            const ElemItem& e = cs[i]->relem[cs[i]->ops[q].first];
+            et.push_back(e);
+            mc = std::max<int64_t>(mc, e.count);
+          }
+          x->launches.push_back({7, x->put(et.data(), et.size()), (int)et.size(), 0, 0, 0, mc, 0, 0, 0});
+          break;
+        }
+        case kOpMaxdiff: {
+          mt.clear();
+          int64_t mc = 0;
+          for (int i : mem) {
+            const MaxdiffItem& m = cs[i]->rmax[cs[i]->ops[q].first];
+            mt.push_back(m);
+            mc = std::max<int64_t>(mc, m.count);
+          }
+          x->launches.push_back({8, x->put(mt.data(), mt.size()), (int)mt.size(), 0, 0, 0, mc, 0, 0, 0});
+          break;
+        }
+      }
+    }
+  }
+  // the step's tables: pinned slot -> device slot, one copy; then the launches
+  const int k = x->slot;
+  x->slot = (x->slot + 1) % Executor::kSlots;
+  if (x->pending[k]) XCHK(x, hipEventSynchronize(x->ev[k]));
+  const size_t need = std::max<size_t>(x->stage.size(), 256);
+  if (need > x->cap[k]) {
+    if (x->host[k]) XCHK(x, hipHostFree(x->host[k]));
+    if (x->devb[k]) XCHK(x, hipFree(x->devb[k]));
+    x->host[k] = nullptr;
+    x->devb[k] = nullptr;
+    const size_t c = std::max<size_t>(need * 2, (size_t)1 << 20);
+    XCHK(x, hipHostMalloc(reinterpret_cast<void**>(&x->host[k]), c, hipHostMallocDefault));
+    XCHK(x, hipMalloc(reinterpret_cast<void**>(&x->devb[k]), c));
+    x->cap[k] = c;
+  }
+  std::memcpy(x->host[k], x->stage.data(), x->stage.size());
+  XCHK(x, hipMemcpyAsync(x->devb[k], x->host[k], x->stage.size(), hipMemcpyHostToDevice, x->st));
+  for (const auto& L : x->launches) CHK(executor_launch(x, L, x->devb[k]));
+  XCHK(x, hipEventRecord(x->ev[k], x->st));
+  x->pending[k] = true;
+  for (int i = 0; i < nc; i++) {
+    Chain* h = cs[i];
+    h->steps++;
+    if (h->steps % (2 * h->M) == 0) h->sweeps++;
+  }
+  return DQMC_OK;
+}
+
+// Runs `nsweeps` sweeps of every chain in lockstep slice visits on one stream: each chain
+// makes exactly its own 2 M nsweeps visits (chains with fewer slices finish earlier).
+int run_lockstep(Chain* const* hs, int count, int nsweeps, int64_t* accepted, std::string& err) {
+  Executor x;
+  int r = executor_init(&x, hs[0]->dev);
+  std::vector<long long> a0(count, 0), target(count, 0), start(count, 0);
+  std::vector<Chain*> att;
+  for (int i = 0; i < count && r == DQMC_OK; i++) {
+    r = executor_attach(&x, hs[i]);
+    if (r == DQMC_OK) {
+      att.push_back(hs[i]);
+      r = read_accepted(hs[i], &a0[i]);
+      start[i] = hs[i]->steps;
+      target[i] = start[i] + 2LL * hs[i]->M * nsweeps;
+    }
+  }
+  std::vector<Chain*> act;
+  while (r == DQMC_OK) {
+    act.clear();
+    for (int i = 0; i < count; i++)
+      if (hs[i]->steps < target[i]) act.push_back(hs[i]);
+    if (act.empty()) break;
+    // leave room in every chain's error slots for this step's stabilisation
+    for (Chain* h : act)
+      if (h->nerr + 2 >= h->errcap && r == DQMC_OK) r = harvest(h);
+    if (r == DQMC_OK) r = executor_step(&x, act.data(), (int)act.size());
+  }
+  if (r != DQMC_OK && err.empty()) err = x.err;
+  int r2 = DQMC_OK;
+  for (Chain* h : att) {
+    int q = executor_detach(&x, h);
+    if (q == DQMC_OK) q = harvest(h);
+    if (q != DQMC_OK && r2 == DQMC_OK) { r2 = q; if (err.empty()) err = h->err; }
+  }
+  if (r == DQMC_OK && accepted)
+    for (int i = 0; i < count && r == DQMC_OK; i++) {
+      long long a1 = 0;
+      r = read_accepted(hs[i], &a1);
+      accepted[i] = a1 - a0[i];
+    }
+  return r != DQMC_OK ? r : r2;
+}
+
 }  // namespace
+
+// Internal interface of the executor for the serving loop (ensemble.h).
+struct dqmc::Ensemble {
+  Executor x;
+};
+
+int dqmc::ensemble_create(int dev, Ensemble** out) {
+  auto* e = new Ensemble();
+  const int r = executor_init(&e->x, dev);
+  if (r != DQMC_OK) { delete e; return r; }
+  *out = e;
+  return DQMC_OK;
+}
+void dqmc::ensemble_destroy(Ensemble* e) { delete e; }
+const char* dqmc::ensemble_error(Ensemble* e) { return e->x.err.c_str(); }
+int dqmc::ensemble_attach(Ensemble* e, dqmc_chain_t h) {
+  if (h->prec != DQMC_PRECISION_FP64 || h->dev != e->x.dev || h->cs < 1 || h->cs > h->M) {
+    e->x.err = "only built FP64 chains of the ensemble's device can join it";
+    return DQMC_E_STATE;
+  }
+  if (hipSetDevice(h->dev) != hipSuccess) return DQMC_E_HIP;
+  return executor_attach(&e->x, h);
+}
+int dqmc::ensemble_detach(Ensemble* e, dqmc_chain_t h) { return executor_detach(&e->x, h); }
+int dqmc::ensemble_step(Ensemble* e, dqmc_chain_t* hs, int count) {
+  for (int i = 0; i < count; i++)
+    if (hs[i]->nerr + 2 >= hs[i]->errcap) {
+      const int r = harvest(hs[i]);
+      if (r != DQMC_OK) { e->x.err = hs[i]->err; return r; }
+    }
+  return executor_step(&e->x, hs, count);
+}
+long long dqmc::chain_steps(dqmc_chain_t h) { return h->steps; }
+int dqmc::chain_slices(dqmc_chain_t h) { return h->M; }
 
 // ======================================================================================
 // C ABI
@@ -1131,12 +1662,6 @@ int dqmc_propagate(dqmc_chain_t h) {
   return harvest(h);
 }
 
-static int read_accepted(Chain* h, long long* acc) {
-  ChainCounters c;
-  HIPCHK(h, hipMemcpy(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost));
-  *acc = c.accepted;
-  return DQMC_OK;
-}
 
 int dqmc_sweep_spatial(dqmc_chain_t h, int64_t* accepted) {
   ENTER(h);
@@ -1172,7 +1697,27 @@ int dqmc_batch_sweep(dqmc_chain_t* hs, int count, int nsweeps, int64_t* accepted
   if (!hs || count < 1 || nsweeps < 0) return DQMC_E_INVALID;
   for (int i = 0; i < count; i++)
     for (int j = 0; j < i; j++)
-      if (!hs[i] || hs[i] == hs[j]) return DQMC_E_INVALID;   // a chain is not shared by two threads
+      if (!hs[i] || hs[i] == hs[j]) return DQMC_E_INVALID;   // a chain appears once
+  bool lockstep = true;
+  for (int i = 0; i < count; i++) {
+    Chain* h = hs[i];
+    if (!h->hopping_set || h->cs < 1 || h->cs > h->M) {
+      h->err = "dqmc_batch_sweep: set the hopping matrix and build the stack first";
+      return DQMC_E_STATE;
+    }
+    if (h->prec != DQMC_PRECISION_FP64 || h->dev != hs[0]->dev || h->prof) lockstep = false;
+  }
+  if (lockstep) {
+    // FP64 chains on one device: one launch per kernel kind covers all of them
+    if (hipSetDevice(hs[0]->dev) != hipSuccess) return DQMC_E_HIP;
+    std::string err;
+    const int r = run_lockstep(hs, count, nsweeps, accepted, err);
+    if (r != DQMC_OK)
+      for (int i = 0; i < count; i++) hs[i]->err = "dqmc_batch_sweep: " + err;
+    return r;
+  }
+  // otherwise (mixed precision, several devices, kernel timing on) one host thread and
+  // stream per chain
   std::vector<int> rc(count, DQMC_OK);
   std::vector<std::thread> th;
   th.reserve(count);
@@ -1247,39 +1792,86 @@ int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* Gout, int ld) {
   ENTER(h);
   NEED_HOPPING(h);
   if (!Gout || ld < h->n || slice < 0 || slice > h->M) { h->err = "dqmc_calculate_greens: bad arguments"; return DQMC_E_INVALID; }
-  const int sm = h->sm, M = h->M;
-  const size_t m = (size_t)h->flv * h->nn;
-  double* ca = h->cur;
-  double* cb = h->tmp2;
-  auto chain_step = [&](double* Ux, double* Dx, double* Tx, bool final) -> int {
-    HIPCHK(h, launch_scale_cols(h->tmp1, ca, Dx, h->n, h->flv, h->st));
-    CHK(udt(h, B(h, h->tmp1).get(), B(h, final ? Ux : ca).get(), Bd(h, Dx).get(), B(h, h->O).get(), 1));
-    CHK(mul_blocks(h, B(h, h->Gx), B(h, h->O), B(h, Tx)));
-    return copyd(h, Tx, h->Gx, m);
-  };
-  CHK(identity(h, ca, h->flv));
-  CHK(udt_identity(h, h->Ur, h->Dr, h->Tr));
-  if (slice + 1 <= M) {
-    for (int k = M; k >= slice + 1; k--) {
-      CHK(slice_dagger_left(h, k, B(h, ca).get(), B(h, cb).get(), nullptr));
-      std::swap(ca, cb);
-      if (k % sm == 0) CHK(chain_step(nullptr, h->Dr, h->Tr, false));
-    }
-    CHK(chain_step(h->Ur, h->Dr, h->Tr, true));
-  }
-  CHK(identity(h, ca, h->flv));
-  CHK(udt_identity(h, h->Ul, h->Dl, h->Tl));
-  if (slice >= 1) {
-    for (int k = 1; k <= slice; k++) {
-      CHK(slice_left(h, k, B(h, ca).get(), B(h, cb).get(), nullptr));
-      std::swap(ca, cb);
-      if (k % sm == 0) CHK(chain_step(nullptr, h->Dl, h->Tl, false));
-    }
-    CHK(chain_step(h->Ul, h->Dl, h->Tl, true));
-  }
-  CHK(calculate_greens(h, h->Gchk));
+  CHK(greens_from_scratch(h, slice, h->Gchk));
   CHK(download_matrix(h, Gout, ld, h->Gchk, h->flv * h->n));
   HIPCHK(h, hipStreamSynchronize(h->st));
+  return DQMC_OK;
+}
+
+// global_update(mc, model, field) with `proposal` as the new conf (global_updates.jl:191-207):
+//   propose_global_from_conf (:136-169): the inverse D of the last calculate_greens of G
+//   (here 1 / Dr) and inv_det at slice current_slice - 1 = 0 (:64-121), the chain from scratch
+//   with the proposal, whose calculate_inv_greens_udt (:25-47) leaves the new D in Dr (the
+//   steps of calculate_greens up to its last UDT); detratio = prod_i (1 / D_old[i]) D_new[i]
+//   in the reference's order, squared for one unique flavour; dE from energy_boson
+//   (fields.jl:395, 451); p = |exp(-dE) detratio|; accepted iff p > 1 || u < p, u from the
+//   chain's uniform stream (consumed only when p <= 1);
+//   accept_global! (:171-188): reverse_build_stack + propagate; otherwise the old conf (and
+//   D) are restored and G is untouched.
+int dqmc_global_update(dqmc_chain_t h, const int8_t* proposal, int ld, int* accepted, double* p_out) {
+  ENTER(h);
+  NEED_HOPPING(h);
+  if (!proposal || ld < h->n) { h->err = "dqmc_global_update: null proposal or ld < n"; return DQMC_E_INVALID; }
+  if (h->cs != 1 || h->dir != 1 || !h->dg_valid) {
+    h->err = "dqmc_global_update: needs current_slice == 1, direction == 1 right after a sweep or a "
+             "stack build (propose_global_from_conf, global_updates.jl:136-140)";
+    return DQMC_E_STATE;
+  }
+  const int n = h->n, M = h->M;
+  const size_t nd = (size_t)h->flv * n, nc = (size_t)n * M;
+  std::vector<int8_t> cnew(nc), cold(nc);
+  for (int l = 0; l < M; l++)
+    for (int i = 0; i < n; i++) {
+      const int8_t x = proposal[(size_t)i + (size_t)l * ld];
+      if (x != 1 && x != -1) { h->err = "dqmc_global_update: entries must be +-1"; return DQMC_E_INVALID; }
+      cnew[(size_t)i + (size_t)l * n] = x;
+    }
+  std::vector<double> dold(nd), dnew(nd);
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(dold.data(), h->Dr, sizeof(double) * nd, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(cold.data(), h->conf, nc, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(h->conf, cnew.data(), nc, hipMemcpyHostToDevice));
+  CHK(greens_from_scratch(h, 0, h->Gchk));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  HIPCHK(h, hipMemcpy(dnew.data(), h->Dr, sizeof(double) * nd, hipMemcpyDeviceToHost));
+  double detratio = 1.0;
+  for (size_t i = 0; i < nd; i++) detratio *= (1.0 / dold[i]) * dnew[i];
+  if (h->flv == 1) detratio = detratio * detratio;
+  double dE = 0.0;
+  if (h->flv == 1) {
+    long long sn = 0, so = 0;
+    for (size_t i = 0; i < nc; i++) { sn += cnew[i]; so += cold[i]; }
+    dE = h->alpha * (double)sn - h->alpha * (double)so;
+  }
+  const double p = std::fabs(std::exp(-dE) * detratio);
+  bool acc = p > 1.0;
+  if (!acc) {
+    // one uniform of the chain's stream (the Metropolis test of local_updates.jl:53 as well)
+    ChainCounters c;
+    HIPCHK(h, hipMemcpy(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost));
+    double u = 1.0;
+    if (h->unif) {
+      if (c.u_pos < h->nunif) HIPCHK(h, hipMemcpy(&u, h->unif + c.u_pos, sizeof(double), hipMemcpyDeviceToHost));
+      else c.exhausted = 1;
+    } else {
+      u = counter_uniform_host(h->seed, (unsigned long long)c.u_pos);
+    }
+    if (std::fabs(u - p) < 1e-12) c.ties += 1;
+    c.u_pos += 1;
+    HIPCHK(h, hipMemcpy(h->ctr, &c, sizeof(c), hipMemcpyHostToDevice));
+    acc = u < p;
+  }
+  if (acc) {
+    CHK(reverse_build(h));
+    CHK(propagate(h));
+    CHK(harvest(h));
+  } else {
+    HIPCHK(h, hipMemcpy(h->conf, cold.data(), nc, hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->Dr, dold.data(), sizeof(double) * nd, hipMemcpyHostToDevice));
+    h->dg_valid = true;
+  }
+  if (accepted) *accepted = acc ? 1 : 0;
+  if (p_out) *p_out = p;
   return DQMC_OK;
 }
 

@@ -46,7 +46,7 @@ constexpr size_t gemm_lds_bytes() { return sizeof(T) * 2 * 2 * kKC * kLD; }
 template <typename T, int TA, int TB>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   typedef typename AccT<T>::type acc_t;
-  const GemmItem<T>& it = batch.it[blockIdx.z];
+  const GemmItem<T>& it = batch.items ? batch.items[blockIdx.z] : batch.it[blockIdx.z];
   const int M = it.M, N = it.N, K = it.K;
   const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup, before any barrier
@@ -200,12 +200,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
 template <typename T>
 static hipError_t launch_gemm_t(const GemmBatch<T>& b, hipStream_t s) {
   if (b.count <= 0) return hipSuccess;
-  if (b.count > kMaxGemmItems) return hipErrorInvalidValue;
   int Mx = 0, Nx = 0;
-  for (int i = 0; i < b.count; i++) {
-    if (b.it[i].K < 1) return hipErrorInvalidValue;
-    if (b.it[i].M > Mx) Mx = b.it[i].M;
-    if (b.it[i].N > Nx) Nx = b.it[i].N;
+  if (b.items) {
+    // item table in device memory (checked by the host code that packed it)
+    if (b.max_m < 1 || b.max_n < 1 || b.count > 65535) return hipErrorInvalidValue;
+    Mx = b.max_m;
+    Nx = b.max_n;
+  } else {
+    if (b.count > kMaxGemmItems) return hipErrorInvalidValue;
+    for (int i = 0; i < b.count; i++) {
+      if (b.it[i].K < 1) return hipErrorInvalidValue;
+      if (b.it[i].M > Mx) Mx = b.it[i].M;
+      if (b.it[i].N > Nx) Nx = b.it[i].N;
+    }
   }
   dim3 grid((Mx + kTM - 1) / kTM, (Nx + kTN - 1) / kTN, b.count);
   dim3 block(256);

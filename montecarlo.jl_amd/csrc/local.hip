@@ -986,6 +986,51 @@ static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
   }
 }
 
+// Many chains per launch: workgroup (0, c) runs chain c's window (when it has one in this
+// launch), workgroups (1 + t, c) apply chain c's previous window to tile t.  Each chain's
+// work is exactly that of its own launch of window_kernel / window_apply_kernel.
+template <typename GT, int FLV, int W>
+__global__ __launch_bounds__(256) void window_multi_kernel(const WindowArgs<GT>* __restrict__ items) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const WindowArgs<GT>& a = items[blockIdx.y];
+  if (blockIdx.x == 0) {
+    if (a.do_window) window_role<GT, FLV, W>(a, smem);
+    return;
+  }
+  if (a.Yp == nullptr) return;
+  const int nt = (a.n + 63) / 64;
+  const int tile = (int)blockIdx.x - 1;
+  if (tile >= FLV * nt * nt) return;
+  apply_tile<GT, W>(a.Gd, a.G, a.Sp, a.Yp, a.n, a.wp, tile, smem);
+}
+
+template <int FLV, int W>
+static void launch_window_multi_w(const WindowArgs<double>* items, int count, int max_n, hipStream_t s) {
+  const int nt = (max_n + 63) / 64;
+  const size_t lds = apply_lds<double, W>() > window_lds<double, FLV, W>() ? apply_lds<double, W>()
+                                                                          : window_lds<double, FLV, W>();
+  hipLaunchKernelGGL((window_multi_kernel<double, FLV, W>), dim3(1 + FLV * nt * nt, count), dim3(256), lds, s,
+                     items);
+}
+
+hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int flv, int W, int max_n,
+                                   hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > 65535 || max_n < 1 || max_n > 2048 || flv < 1 || flv > 2) return hipErrorInvalidValue;
+  if (flv == 1) {
+    if (W == 16) launch_window_multi_w<1, 16>(items, count, max_n, s);
+    else if (W == 32) launch_window_multi_w<1, 32>(items, count, max_n, s);
+    else if (W == 64) launch_window_multi_w<1, 64>(items, count, max_n, s);
+    else return hipErrorInvalidValue;
+  } else {
+    if (W == 16) launch_window_multi_w<2, 16>(items, count, max_n, s);
+    else if (W == 32) launch_window_multi_w<2, 32>(items, count, max_n, s);
+    else if (W == 64) launch_window_multi_w<2, 64>(items, count, max_n, s);
+    else return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <typename GT>
 static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
   if (a.flv < 1 || a.flv > 2 || (a.W != 16 && a.W != 32 && a.W != 64)) return hipErrorInvalidValue;
@@ -1024,7 +1069,13 @@ hipError_t local_kernels_init() {
                        reinterpret_cast<const void*>(window_apply_kernel<double, 64>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 32>),
-                       reinterpret_cast<const void*>(window_apply_kernel<float, 64>)};
+                       reinterpret_cast<const void*>(window_apply_kernel<float, 64>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 16>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 32>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 64>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 16>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 32>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 64>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;

@@ -87,8 +87,9 @@ __host__ __device__ static int udt_panel_width(int rows) {
 }
 
 __global__ __launch_bounds__(256) void udt_norms_kernel(UdtBatch batch) {
-  const UdtItem it = batch.it[blockIdx.y];
-  const int n = batch.n;
+  const bool tab = batch.items != nullptr;
+  const UdtItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
+  const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int col = blockIdx.x * 4 + wave;
   if (col >= n) return;  // per-wave work only: no workgroup barrier in this kernel
@@ -346,9 +347,11 @@ __device__ __forceinline__ void panel_steps_block(double (&a)[R][WM], double* re
 // Per column step one block reduction gives ||x||^2 and x^T a_c for the later columns:
 // with v = (x + nu e_j) / (x_j + nu), v^T a_c = a_jc + (x_{>j}^T a_c{>j}) / (x_j + nu).
 template <int R, int WM>
-__global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, int p0, int w, int last) {
-  const UdtItem it = batch.it[blockIdx.x];
-  const int n = batch.n;
+__global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, int p0a, int wa, int lasta) {
+  const bool tab = batch.items != nullptr;
+  const UdtItem it = tab ? batch.items[blockIdx.x] : batch.it[blockIdx.x];
+  const int n = tab ? it.n : batch.n;
+  const int p0 = tab ? it.p0 : p0a, w = tab ? it.w : wa, last = tab ? it.last : lasta;
   const int rows = n - p0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       const unsigned long long hold = __ballot(slot >= 0 && cont == c);
       // every choice is held by exactly one tracked slot; an empty ballot means a broken
       // selection: the swap is skipped and the event counted as a device error
-      if (!hold && lane == 0 && batch.err) atomicAdd(batch.err, 1ull);
+      if (!hold && lane == 0 && it.err) atomicAdd(it.err, 1ull);
       const int src = hold ? __ffsll((long long)hold) - 1 : t;  // lane whose slot holds c
       const int a_cont = __builtin_amdgcn_readlane(cont, t);      // content of slot p0+t
       const int b_cont = __builtin_amdgcn_readlane(cont, src);
@@ -783,6 +786,16 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
   __shared__ double red[4][4][64];
   __shared__ double nred[4][16];
   const int bx = blockIdx.x;
+  if (batch.items) {
+    // item table: this item's own panel; workgroups beyond its strips leave at once
+    const UdtItem& it = batch.items[blockIdx.y];
+    const int n = it.n, c0i = it.p0 + it.w;
+    const int nAi = c0i < n ? (n - c0i + 15) / 16 : 0, nQi = (n + 15) / 16;
+    if (bx >= nAi + nQi) return;
+    if (bx < nAi) reflect_strip(it, n, it.p0, it.w, c0i + 16 * bx, 0, red, nred);
+    else reflect_strip(it, n, it.p0, it.w, 16 * (bx - nAi), 2, red, nred);
+    return;
+  }
   if (bx < nA) reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * bx, 0, red, nred);
   else reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, 16 * (bx - nA), 2, red, nred);
 }
@@ -790,8 +803,9 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
 // is copied), one wave per column.
 __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
-  const UdtItem it = batch.it[blockIdx.y];
-  const int n = batch.n;
+  const bool tab = batch.items != nullptr;
+  const UdtItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
+  const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = blockIdx.x * 4 + wave;
   if (j >= n) return;
@@ -817,6 +831,19 @@ __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
 }
 
 
+int udt_plan_fixed(int n, int* starts, int* widths, int cap) {
+  int np = 0;
+  for (int p0 = 0; p0 < n;) {
+    const int w = udt_panel_width(n - p0);
+    if (np == cap) return -1;
+    starts[np] = p0;
+    widths[np] = w;
+    np++;
+    p0 += w;
+  }
+  return np;
+}
+
 void udt_plan(int n, std::vector<int>& starts, std::vector<int>& widths) {
   starts.clear();
   widths.clear();
@@ -839,7 +866,8 @@ hipError_t udt_check(const UdtBatch& b) {
 }
 
 hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s) {
-  hipLaunchKernelGGL(udt_norms_kernel, dim3((b.n + 3) / 4, b.count), dim3(256), 0, s, b);
+  const int n = b.items ? b.max_n : b.n;
+  hipLaunchKernelGGL(udt_norms_kernel, dim3((n + 3) / 4, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
@@ -871,7 +899,31 @@ hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, hipStream_t s) {
 }
 
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s) {
-  hipLaunchKernelGGL(udt_finish_kernel, dim3((b.n + 3) / 4, b.count), dim3(256), 0, s, b);
+  const int n = b.items ? b.max_n : b.n;
+  hipLaunchKernelGGL(udt_finish_kernel, dim3((n + 3) / 4, b.count), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+int udt_rows_class(int rows) { return udt_rows_per_thread(rows); }
+int udt_width_for(int rows) { return udt_panel_width(rows); }
+
+hipError_t launch_udt_panel_tab(const UdtBatch& b, int R, hipStream_t s) {
+  if (!b.items || b.count < 1 || b.max_n < 1 || b.max_n > 2048) return hipErrorInvalidValue;
+  UdtBatch c = b;
+  c.n = b.max_n;          // sizes the dynamic LDS (per-item arrays of n entries)
+  switch (R) {
+    case 1: launch_panel_t<1, 16>(c, 0, 0, 0, s); break;
+    case 2: launch_panel_t<2, 16>(c, 0, 0, 0, s); break;
+    case 4: launch_panel_t<4, 16>(c, 0, 0, 0, s); break;
+    case 8: launch_panel_t<8, 8>(c, 0, 0, 0, s); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_udt_reflect_tab(const UdtBatch& b, int max_grid, hipStream_t s) {
+  if (!b.items || b.count < 1 || max_grid < 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(udt_reflect_kernel, dim3(max_grid, b.count), dim3(256), 0, s, b, 0, 0, 0, 0);
   return hipGetLastError();
 }
 
@@ -897,8 +949,9 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
 // the matrix cores, then the 16 x 16 triangular block is solved by 16 lanes (one row each)
 // in the reference's order: x_j = (o_j - sum_{k<j} x_k T_kj) / T_jj.
 __global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
-  const RdivpItem it = batch.it[blockIdx.y];
-  const int n = batch.n;
+  const bool tab = batch.items != nullptr;
+  const RdivpItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
+  const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * 64 + wave * 16;
@@ -959,15 +1012,15 @@ __global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
 // rdivp for n <= kRdivpStripMaxN: one wave per 16-row strip.  The strip's pivoted input
 // is gathered into LDS once (X[k][r], column-major per strip) and every solved column
 // stays there, so a panel's coupling X_{<J} T_{<J,J} reads only T from global memory.
-constexpr int kRdivpStripMaxN = 1024;
-
 static size_t rdivp_strip_lds_bytes(int n) { return ((size_t)n * 17 + 2 * 16 * 17) * 8 + (size_t)n * 4; }
 
 __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
-  const RdivpItem it = batch.it[blockIdx.y];
-  const int n = batch.n;
+  const bool tab = batch.items != nullptr;
+  const RdivpItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
+  const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * 16;
+  if (r0 >= n) return;   // item-table launches size the grid for the largest item
   extern __shared__ __align__(16) unsigned char smem[];
   double* X = reinterpret_cast<double*>(smem);   // [n][17]
   double* Ysh = X + (size_t)n * 17;              // [16][17]
@@ -1102,6 +1155,17 @@ __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
 
 hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
   if (b.count <= 0) return hipSuccess;
+  if (b.items) {
+    // item table: one kernel class per launch (the host keeps n <= kRdivpStripMaxN items
+    // and larger ones apart); the grid and LDS are sized for the largest item
+    if (b.max_n < 1 || b.max_n > 2048) return hipErrorInvalidValue;
+    if (b.max_n <= kRdivpStripMaxN)
+      hipLaunchKernelGGL(rdivp_strip_kernel, dim3((b.max_n + 15) / 16, b.count), dim3(64),
+                         rdivp_strip_lds_bytes(b.max_n), s, b);
+    else
+      hipLaunchKernelGGL(rdivp_kernel, dim3((b.max_n + 63) / 64, b.count), dim3(256), 0, s, b);
+    return hipGetLastError();
+  }
   if (b.count > kMaxUdtItems) return hipErrorInvalidValue;
   if (b.n <= kRdivpStripMaxN)
     hipLaunchKernelGGL(rdivp_strip_kernel, dim3((b.n + 15) / 16, b.count), dim3(64), rdivp_strip_lds_bytes(b.n), s,

@@ -6,12 +6,17 @@
 // matrix, initial field, counter-stream seed and sweep count.  A scheduler thread admits
 // queued requests in arrival order while the number of running chains is below
 // max_active and the sum of their footprints (dqmc_footprint_bytes: the UDT stack and
-// every other device buffer of the chain) fits the memory budget.  Each admitted request
-// runs on its own host thread and HIP stream: create, set hopping / field / seed,
-// build_stack, sweep, read back field, G and statistics, destroy.  Requests that arrive
-// while others run are admitted as soon as a slot and memory free up, so their kernels
-// overlap with the running ones.  A chain's result does not depend on what else runs:
-// every kernel of a chain reads and writes only that chain's buffers, in its stream order.
+// every other device buffer of the chain) fits the memory budget.
+//   - An admitted request is prepared on a thread of its own (create, set hopping / field /
+//     seed, build_stack on the chain's own stream), then joins the executor.
+//   - The executor thread runs lockstep slice visits (sweep_spatial + propagate) of every
+//     FP64 chain that has joined, one launch per kernel kind covering all of them
+//     (ensemble.h); a chain leaves after its 2 M nsweeps visits and a finishing thread reads
+//     back field, G and statistics and frees it.  Requests that arrive while others run
+//     join at the next step, so their work shares the running launches.
+//   - Mixed-precision requests run on their own thread and stream (dqmc_sweep).
+// A chain's result does not depend on what else runs: every kernel of a chain reads and
+// writes only that chain's buffers, in the chain's own order.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -23,6 +28,7 @@
 #include <vector>
 
 #include "../../include/dqmc_hip.h"
+#include "ensemble.h"
 
 namespace {
 
@@ -47,6 +53,10 @@ struct Request {
   dqmc_stats_t stats{};
   int64_t accepted = 0;
   double t_submit = 0, t_start = 0, t_done = 0;
+  // executor state of a running request (touched by its own threads and the executor only)
+  dqmc_chain_t h = nullptr;
+  int64_t acc0 = 0;
+  long long target = 0;
 };
 
 }  // namespace
@@ -64,6 +74,10 @@ struct dqmc_server_s {
   int waiters = 0;        // threads inside dqmc_server_wait (destroy waits for them to leave)
   bool stop = false;
   std::thread scheduler;
+  // the executor: built chains wait in `joining`, then run in lockstep steps
+  std::thread executor;
+  std::deque<std::shared_ptr<Request>> joining;
+  int helpers = 0;        // preparing / finishing threads still running
 };
 
 namespace {
@@ -72,33 +86,131 @@ namespace {
 // never write the same string.
 thread_local std::string t_server_error;
 
-// One request, start to finish, on the calling (worker) thread; false on failure (r.err).
-bool run_request(Request& r, int device) {
+// Prepares a request's chain on the calling thread: create, hopping, field, seed and the
+// stack build; false on failure (r.err).
+bool prepare(Request& r, int device) {
   dqmc_config_t c = r.cfg;
   c.device = device;
-  dqmc_chain_t h = nullptr;
+  auto fail = [&](const char* what) {
+    r.err = std::string(what) + ": " + dqmc_last_error(r.h);
+    return false;
+  };
+  if (dqmc_create(&c, &r.h) != DQMC_OK) return fail("dqmc_create");
+  const int n = c.n_sites;
+  if (dqmc_set_hopping(r.h, r.T.data(), n) != DQMC_OK) return fail("dqmc_set_hopping");
+  if (dqmc_set_field(r.h, r.conf.data(), n) != DQMC_OK) return fail("dqmc_set_field");
+  if (dqmc_set_seed(r.h, r.seed, 0) != DQMC_OK) return fail("dqmc_set_seed");
+  if (dqmc_build_stack(r.h) != DQMC_OK) return fail("dqmc_build_stack");
+  dqmc_stats_t st{};
+  if (dqmc_get_stats(r.h, &st) != DQMC_OK) return fail("dqmc_get_stats");
+  r.acc0 = st.accepted;
+  return true;
+}
+
+// Reads back the results of a finished chain (on its own stream again) and frees it.
+bool finish(Request& r) {
+  dqmc_chain_t h = r.h;
   auto fail = [&](const char* what) {
     r.err = std::string(what) + ": " + dqmc_last_error(h);
     return false;
   };
-  auto body = [&]() -> bool {
-    if (dqmc_create(&c, &h) != DQMC_OK) return fail("dqmc_create");
-    const int n = c.n_sites;
-    if (dqmc_set_hopping(h, r.T.data(), n) != DQMC_OK) return fail("dqmc_set_hopping");
-    if (dqmc_set_field(h, r.conf.data(), n) != DQMC_OK) return fail("dqmc_set_field");
-    if (dqmc_set_seed(h, r.seed, 0) != DQMC_OK) return fail("dqmc_set_seed");
-    if (dqmc_build_stack(h) != DQMC_OK) return fail("dqmc_build_stack");
-    if (dqmc_sweep(h, r.nsweeps, &r.accepted) != DQMC_OK) return fail("dqmc_sweep");
-    r.conf_out.assign((size_t)n * c.slices, 0);
-    if (dqmc_get_field(h, r.conf_out.data(), n) != DQMC_OK) return fail("dqmc_get_field");
-    r.G_out.assign((size_t)n * n * c.flavors, 0.0);
-    if (dqmc_get_greens(h, r.G_out.data(), n, 0) != DQMC_OK) return fail("dqmc_get_greens");
-    if (dqmc_get_stats(h, &r.stats) != DQMC_OK) return fail("dqmc_get_stats");
-    return true;
-  };
-  const bool ok = body();
-  if (h) dqmc_destroy(h);
+  const int n = r.cfg.n_sites;
+  bool ok = true;
+  if (dqmc_get_stats(h, &r.stats) != DQMC_OK) ok = fail("dqmc_get_stats");
+  else if (r.stats.device_errors > 0) ok = fail("device error");
+  if (ok) {
+    r.accepted = r.stats.accepted - r.acc0;
+    r.conf_out.assign((size_t)n * r.cfg.slices, 0);
+    if (dqmc_get_field(h, r.conf_out.data(), n) != DQMC_OK) ok = fail("dqmc_get_field");
+  }
+  if (ok) {
+    r.G_out.assign((size_t)n * n * r.cfg.flavors, 0.0);
+    if (dqmc_get_greens(h, r.G_out.data(), n, 0) != DQMC_OK) ok = fail("dqmc_get_greens");
+  }
+  dqmc_destroy(h);
+  r.h = nullptr;
   return ok;
+}
+
+// A request on its own thread and stream, start to finish (mixed precision).
+bool run_alone(Request& r, int device) {
+  bool ok = prepare(r, device);
+  if (ok && dqmc_sweep(r.h, r.nsweeps, nullptr) != DQMC_OK) {
+    r.err = std::string("dqmc_sweep: ") + dqmc_last_error(r.h);
+    ok = false;
+  }
+  if (ok) return finish(r);
+  if (r.h) { dqmc_destroy(r.h); r.h = nullptr; }
+  return false;
+}
+
+void complete(dqmc_server_s* s, const std::shared_ptr<Request>& r, bool ok) {
+  std::lock_guard<std::mutex> g(s->mu);
+  r->state = ok ? kDone : kFailed;
+  r->t_done = now_s();
+  s->active--;
+  s->used_bytes -= r->footprint;
+  s->helpers--;
+  s->cv.notify_all();
+}
+
+void executor_loop(dqmc_server_s* s) {
+  dqmc::Ensemble* ens = nullptr;
+  const int rc = dqmc::ensemble_create(s->cfg.device, &ens);
+  std::vector<std::shared_ptr<Request>> act;
+  std::vector<dqmc_chain_t> hs;
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&]() { return !act.empty() || !s->joining.empty() || (s->stop && s->helpers == 0); });
+      if (act.empty() && s->joining.empty() && s->stop && s->helpers == 0) break;
+      while (!s->joining.empty()) {
+        act.push_back(s->joining.front());
+        s->joining.pop_front();
+      }
+    }
+    // newly joined chains attach (a failed attach fails the request)
+    for (size_t i = 0; i < act.size();) {
+      Request& r = *act[i];
+      if (r.target == 0) {
+        if (rc != DQMC_OK || dqmc::ensemble_attach(ens, r.h) != DQMC_OK) {
+          r.err = rc != DQMC_OK ? "executor could not start" : dqmc::ensemble_error(ens);
+          dqmc_destroy(r.h);
+          r.h = nullptr;
+          complete(s, act[i], false);
+          act.erase(act.begin() + i);
+          continue;
+        }
+        r.target = dqmc::chain_steps(r.h) + 2LL * dqmc::chain_slices(r.h) * r.nsweeps;
+      }
+      i++;
+    }
+    hs.clear();
+    for (auto& r : act) hs.push_back(r->h);
+    bool step_ok = true;
+    std::string step_err;
+    if (!hs.empty() && dqmc::ensemble_step(ens, hs.data(), (int)hs.size()) != DQMC_OK) {
+      step_ok = false;
+      step_err = dqmc::ensemble_error(ens);
+    }
+    // chains that made all their visits (or all, after a failed step) leave and finish on
+    // threads of their own
+    for (size_t i = 0; i < act.size();) {
+      std::shared_ptr<Request> r = act[i];
+      if (step_ok && dqmc::chain_steps(r->h) < r->target) { i++; continue; }
+      act.erase(act.begin() + i);
+      const bool left = dqmc::ensemble_detach(ens, r->h) == DQMC_OK;
+      if (!step_ok) r->err = "lockstep step failed: " + step_err;
+      const bool ok = step_ok && left;
+      std::thread([s, r, ok]() {
+        bool fine = ok;
+        if (fine) fine = finish(*r);
+        else if (r->h) { dqmc_destroy(r->h); r->h = nullptr; }
+        complete(s, r, fine);
+      }).detach();
+    }
+  }
+  if (ens) dqmc::ensemble_destroy(ens);
 }
 
 void scheduler_loop(dqmc_server_s* s) {
@@ -116,16 +228,26 @@ void scheduler_loop(dqmc_server_s* s) {
     s->active++;
     if (s->active > s->peak_active) s->peak_active = s->active;
     s->used_bytes += r->footprint;
+    s->helpers++;
     r->state = kRunning;
     r->t_start = now_s();
     const int device = s->cfg.device;
     std::thread([s, r, device]() {
-      const bool ok = run_request(*r, device);
+      if (r->cfg.precision != DQMC_PRECISION_FP64) {
+        complete(s, r, run_alone(*r, device));
+        return;
+      }
+      if (!prepare(*r, device)) {
+        if (r->h) { dqmc_destroy(r->h); r->h = nullptr; }
+        complete(s, r, false);
+        return;
+      }
+      if (r->nsweeps == 0) {
+        complete(s, r, finish(*r));
+        return;
+      }
       std::lock_guard<std::mutex> g(s->mu);
-      r->state = ok ? kDone : kFailed;
-      r->t_done = now_s();
-      s->active--;
-      s->used_bytes -= r->footprint;
+      s->joining.push_back(r);   // the executor owns it now; `helpers` drops when it finishes
       s->cv.notify_all();
     }).detach();
   }
@@ -149,6 +271,7 @@ int dqmc_server_create(const dqmc_server_config_t* cfg, dqmc_server_t* out) {
   auto* s = new dqmc_server_s();
   s->cfg = *cfg;
   s->scheduler = std::thread(scheduler_loop, s);
+  s->executor = std::thread(executor_loop, s);
   *out = s;
   return DQMC_OK;
 }
@@ -272,9 +395,10 @@ int dqmc_server_destroy(dqmc_server_t s) {
     s->queue.clear();
     s->stop = true;
     s->cv.notify_all();
-    s->cv.wait(lk, [&]() { return s->active == 0 && s->waiters == 0; });
+    s->cv.wait(lk, [&]() { return s->active == 0 && s->waiters == 0 && s->helpers == 0; });
   }
   s->scheduler.join();
+  s->executor.join();
   delete s;
   return DQMC_OK;
 }

@@ -194,6 +194,63 @@ hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b,
   return hipGetLastError();
 }
 
+// Many-chain forms: item k of the table is handled by the workgroups with blockIdx.y = k.
+__global__ __launch_bounds__(256) void elem_multi_kernel(const ElemItem* __restrict__ items) {
+  const ElemItem& it = items[blockIdx.y];
+  const int64_t count = it.count;
+  double* __restrict__ dst = it.dst;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  if (it.op == kElemCopy) {
+    const double* __restrict__ src = it.src;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < count; i += stride) dst[i] = src[i];
+  } else if (it.op == kElemIdentity) {
+    const int64_t nn = (int64_t)it.n * it.n;
+    const int n = it.n;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < count; i += stride) {
+      const int64_t e = i % nn;
+      dst[i] = (e % n) == (e / n) ? 1.0 : 0.0;
+    }
+  } else {
+    const double v = it.v;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < count; i += stride) dst[i] = v;
+  }
+}
+hipError_t launch_elem_multi(const ElemItem* items, int count, int64_t max_count, hipStream_t s) {
+  if (count <= 0 || max_count <= 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  int64_t g = (max_count + 255) / 256;
+  if (g > 256) g = 256;
+  hipLaunchKernelGGL(elem_multi_kernel, dim3((unsigned)g, count), dim3(256), 0, s, items);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void maxabsdiff_multi_kernel(const MaxdiffItem* __restrict__ items) {
+  __shared__ double part[4];
+  const MaxdiffItem& it = items[blockIdx.y];
+  const double* __restrict__ a = it.a;
+  const double* __restrict__ b = it.b;
+  const int64_t count = it.count;
+  double m = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < count; i += (int64_t)gridDim.x * 256)
+    m = nan_max(m, fabs(a[i] - b[i]));
+  for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double r = nan_max(nan_max(part[0], part[1]), nan_max(part[2], part[3]));
+    atomicMax(reinterpret_cast<unsigned long long*>(it.out), (unsigned long long)__double_as_longlong(r));
+  }
+}
+hipError_t launch_maxabsdiff_multi(const MaxdiffItem* items, int count, int64_t max_count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (count > 65535) return hipErrorInvalidValue;
+  int64_t g = (max_count + 255) / 256;
+  if (g > 128) g = 128;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(maxabsdiff_multi_kernel, dim3((unsigned)g, count), dim3(256), 0, s, items);
+  return hipGetLastError();
+}
+
 // max column sum |A| (1-norm), used to pick the scaling of the matrix exponential
 __global__ __launch_bounds__(1024) void norm1_kernel(const double* __restrict__ A, int n, double* out) {
   __shared__ double part[16];

@@ -26,6 +26,7 @@ import numpy as np
 
 from .parameters import DQMCParameters
 from .measurements import MeasurementSet
+from .updates import AcceptanceStatistics, LocalSweep, SimpleScheduler
 
 
 class ExitCode(enum.IntEnum):
@@ -68,28 +69,12 @@ class DQMCAnalysis:
     me_runtime: float = 0.0
 
 
-@dataclass
-class AcceptanceStatistics:
-    accepted: float = 0.0
-    total: int = 0
-
-    @property
-    def acceptance_rate(self) -> float:
-        """accepted / max(1, total), as max_acceptance reads it (scheduler.jl:291-299)."""
-        return self.accepted / max(1, self.total)
-
-
-class LocalSweep:
-    """One sweep of local updates: 2M slice visits x N proposals (local_updates.jl:70-84)."""
-    name = "LocalSweep"
-
-
 class DQMC:
     def __init__(self, model, *, seed: int = -1, thermalization: int = 100, sweeps: int = 100,
                  measure_rate: int = 10, last_sweep: int = 0, field=None, precision: str = "fp64",
                  device: int = 0, chain_factory=None, silent: bool = False,
                  device_measurements: bool = False, recorder=None, recording_rate: int | None = None,
-                 **kwargs):
+                 scheduler: SimpleScheduler | None = None, **kwargs):
         from .fileio import ConfigRecorder
         self.parameters = DQMCParameters.create(thermalization=thermalization, sweeps=sweeps,
                                                 measure_rate=measure_rate, silent=silent, **kwargs)
@@ -103,8 +88,9 @@ class DQMC:
         self.last_sweep = last_sweep
         self.precision = precision
         self.device = device
-        self.scheduler = [LocalSweep()]
-        self.acceptance = AcceptanceStatistics()
+        self.scheduler = scheduler if scheduler is not None else SimpleScheduler(LocalSweep())
+        # proposals of the shuffling global moves (Julia's RNG is not reproducible outside Julia)
+        self.rng = np.random.default_rng(self.seed)
         self.analysis = DQMCAnalysis()
         self.recorder = recorder if recorder is not None else \
             ConfigRecorder(measure_rate if recording_rate is None else recording_rate)
@@ -137,6 +123,11 @@ class DQMC:
         self.stack.set_hopping(self.model.hopping_matrix())
         self.stack.set_field(self.field.conf)
         self.stack.set_seed(self.stream_seed, self.rng_position)
+
+    @property
+    def acceptance(self) -> AcceptanceStatistics:
+        """Acceptance statistics of the first full-sweep update (the LocalSweep)."""
+        return next(s for s in self.scheduler.sequence if s.is_full_sweep)
 
     def sync_from_device(self):
         """Field and RNG position of the device chain -> this object (before saves)."""
@@ -206,8 +197,8 @@ class DQMC:
         fail_filename = fail_filename or f"failed_{tag}.npz"
 
         self.initialize_run()
+        self.scheduler.init(self)
         total = p.thermalization + p.sweeps
-        n, M = len(self.model.l), p.slices
         t0 = time.perf_counter()
         last_checkpoint = time.time()
         min_sweeps = round(1 / min_update_rate)
@@ -215,11 +206,7 @@ class DQMC:
         exit_code = ExitCode.SUCCESS
         while self.last_sweep < total:
             ts = time.perf_counter()
-            acc = self.stack.sweep(1)
-            self.last_sweep += 1
-            rate = acc / (2 * n * M)
-            self.acceptance.accepted += rate
-            self.acceptance.total += 1
+            self.scheduler.update(self)        # updates up to a full sweep; last_sweep += 1
             if self.last_sweep <= p.thermalization:
                 if self.last_sweep == p.thermalization:
                     self.analysis.th_runtime += time.perf_counter() - t0
@@ -229,19 +216,21 @@ class DQMC:
                 if self.last_sweep % p.measure_rate == 0:
                     self._measure()
             max_sweep_duration = max(max_sweep_duration, time.perf_counter() - ts)
-            # cancel if updates are not accepted (DQMC.jl:289-302): the cumulative acceptance
-            # of the update, checked once more than round(1 / min_update_rate) sweeps ran
-            if self.last_sweep > min_sweeps and self.acceptance.acceptance_rate < min_update_rate:
+            # cancel if updates are not accepted (DQMC.jl:289-302): the largest cumulative
+            # acceptance over the updates (max_acceptance, scheduler.jl:291-299), checked once
+            # more than round(1 / min_update_rate) sweeps ran
+            if self.last_sweep > min_sweeps and self.scheduler.max_acceptance() < min_update_rate:
                 if verbose:
                     print("Cancelling Simulation due to low acceptance rate:")
-                    print(f"\t{self.last_sweep}: {self.acceptance.acceptance_rate}")
+                    print(f"\t{self.last_sweep}: {self.scheduler.max_acceptance()}")
                 save(fail_filename, self, overwrite=overwrite, rename=False)
                 if overwrite and os.path.isfile(resumable_filename):
                     os.remove(resumable_filename)
                 exit_code = ExitCode.CANCELLED_LOW_ACCEPTANCE
                 break
             if verbose and self.last_sweep % p.print_rate == 0:
-                print(f"[{self.last_sweep}/{total}] acceptance {rate:.3f}")
+                stats = ", ".join(f"{s.name} {s.acceptance_rate:.3f}" for s in self.scheduler.sequence)
+                print(f"[{self.last_sweep}/{total}] acceptance: {stats}")
             # early save before the time limit (DQMC.jl:326-343)
             if deadline - time.time() < grace + 2.0 * max_sweep_duration:
                 if verbose:

@@ -127,10 +127,19 @@ def _entries(mc) -> dict:
     put("MC/Model/l/name", m.l.name)
     put("MC/Model/l/Ls", np.array(m.l.Ls, dtype=np.int64))
     put("MC/Scheduler/tag", "SimpleScheduler")
-    put("MC/Scheduler/sequence/1/tag", "AcceptanceStatistics")
-    put("MC/Scheduler/sequence/1/accepted", mc.acceptance.accepted)
-    put("MC/Scheduler/sequence/1/total", mc.acceptance.total)
-    put("MC/Scheduler/sequence/1/update/tag", "LocalSweep")
+    put("MC/Scheduler/idx", mc.scheduler.idx)
+    put("MC/Scheduler/length", len(mc.scheduler.sequence))
+    for k, st in enumerate(mc.scheduler.sequence, start=1):
+        put(f"MC/Scheduler/sequence/{k}/tag", "AcceptanceStatistics")
+        put(f"MC/Scheduler/sequence/{k}/accepted", st.accepted)
+        put(f"MC/Scheduler/sequence/{k}/total", st.total)
+        put(f"MC/Scheduler/sequence/{k}/update/tag", st.update.name)
+    # the numpy generator of the shuffling global moves (PCG64: 128-bit state and increment)
+    st = mc.rng.bit_generator.state
+    put("MC/rng/state", np.frombuffer(st["state"]["state"].to_bytes(16, "little"), dtype=np.uint8))
+    put("MC/rng/inc", np.frombuffer(st["state"]["inc"].to_bytes(16, "little"), dtype=np.uint8))
+    put("MC/rng/has_uint32", st["has_uint32"])
+    put("MC/rng/uinteger", st["uinteger"])
     for key, ms in mc.measurements.items():
         for name, arr in ms.state().items():
             put(f"MC/Measurements/ME/{key}/{name}", arr)
@@ -194,6 +203,16 @@ def load(filename: str, **dqmc_kwargs):
     model = HubbardModel(l=l, U=float(d["MC/Model/U"]), mu=float(d["MC/Model/mu"]),
                          t=float(d["MC/Model/t"]))
     P = lambda k: d[f"MC/Parameters/{k}"]
+    from . import updates as U
+    seq = []
+    for k in range(1, int(d["MC/Scheduler/length"]) + 1):
+        name = str(d[f"MC/Scheduler/sequence/{k}/update/tag"])
+        seq.append(getattr(U, name)())
+    scheduler = U.SimpleScheduler(*seq)
+    scheduler.idx = int(d["MC/Scheduler/idx"])
+    for k, st in enumerate(scheduler.sequence, start=1):
+        st.accepted = float(d[f"MC/Scheduler/sequence/{k}/accepted"])
+        st.total = int(d[f"MC/Scheduler/sequence/{k}/total"])
     recorder = ConfigRecorder(int(d["MC/configs/rate"])) if str(d["MC/configs/tag"]) == "ConfigRecorder" \
         else Discarder()
     mc = DQMC(model, seed=int(d["RNG/seed"]), thermalization=int(P("thermalization")),
@@ -203,7 +222,7 @@ def load(filename: str, **dqmc_kwargs):
               check_propagation_error=bool(P("check_propagation_error")),
               safe_mult=int(P("safe_mult")), print_rate=int(P("print_rate")),
               delta_tau=float(P("delta_tau")), beta=float(P("beta")), slices=int(P("slices")),
-              precision=str(d["MC/precision"]), recorder=recorder,
+              precision=str(d["MC/precision"]), recorder=recorder, scheduler=scheduler,
               device_measurements=bool(d["MC/device_measurements"]), **dqmc_kwargs)
     if mc.field.name != str(d["MC/field/name"]):
         raise ValueError(f"field {d['MC/field/name']} does not match the model's {mc.field.name}")
@@ -221,8 +240,12 @@ def load(filename: str, **dqmc_kwargs):
         s.max = float(d[f"MC/Analysis/{name}/max"])
         s.sum = float(d[f"MC/Analysis/{name}/sum"])
         setattr(a, name, s)
-    mc.acceptance.accepted = float(d["MC/Scheduler/sequence/1/accepted"])
-    mc.acceptance.total = int(d["MC/Scheduler/sequence/1/total"])
+    st = mc.rng.bit_generator.state
+    st["state"]["state"] = int.from_bytes(bytes(d["MC/rng/state"]), "little")
+    st["state"]["inc"] = int.from_bytes(bytes(d["MC/rng/inc"]), "little")
+    st["has_uint32"] = int(d["MC/rng/has_uint32"])
+    st["uinteger"] = int(d["MC/rng/uinteger"])
+    mc.rng.bit_generator.state = st
     prefix = "MC/Measurements/ME/"
     groups: dict[str, dict] = {}
     for k, v in d.items():

@@ -905,6 +905,51 @@ EXPORT void orc_calculate_greens_slice(oracle_t *o, int slice, double *out) {
 }
 
 /* greens(mc): e^{+dtau T/2} G e^{-dtau T/2}, greens.jl:114-125 */
+/* global_update(mc, model, field) (global_updates.jl:191-207) for the proposal `proposal`
+ * (the new field.conf; the old one plays temp_conf):
+ *   propose_global_from_conf (:136-169): tempvf = Dl, the inverse D of the last
+ *     calculate_greens; inv_det(mc, current_slice - 1 = 0) (:64-121): the chain from scratch
+ *     at slice 0 with the proposal, whose calculate_inv_greens_udt (:25-47) leaves the new D
+ *     in Dr (the same steps as calculate_greens up to its last UDT); detratio = prod over
+ *     all entries of tempvf[i] * Dr[i], squared for one unique flavour; dE = E_b(new) - E_b(old)
+ *     with energy_boson = alpha sum(conf) (density) or 0 (magnetic) (fields.jl:395, 451);
+ *   p = |exp(-dE) detratio|; accepted iff p > 1 || rand() < p (a uniform only when p <= 1);
+ *   accept_global! (:171-188): reverse_build_stack + propagate; otherwise the old conf is
+ *   restored.
+ * Requires current_slice == 1 and direction == 1 (the state after a sweep or a build).
+ * Returns 1 if accepted, 0 if rejected, -1 if the state is wrong; *p_out = p. */
+EXPORT int orc_global_update(oracle_t *o, const int8_t *proposal, double *p_out) {
+    int n = o->n, M = o->M, flv = o->flv;
+    if (o->current_slice != 1 || o->direction != 1) return -1;
+    double *tempvf = malloc(sizeof(double) * flv * n);
+    int8_t *old = malloc((size_t)n * M);
+    memcpy(tempvf, o->Dl, sizeof(double) * flv * n);
+    memcpy(old, o->conf, (size_t)n * M);
+    memcpy(o->conf, proposal, (size_t)n * M);
+    orc_calculate_greens_slice(o, 0, o->greens_temp);
+    double detratio = 1.0;
+    for (int i = 0; i < flv * n; i++) detratio *= tempvf[i] * o->Dr[i];
+    if (flv == 1) detratio = detratio * detratio;
+    double dE = 0.0;
+    if (flv == 1) {
+        int64_t s_new = 0, s_old = 0;
+        for (size_t i = 0; i < (size_t)n * M; i++) { s_new += o->conf[i]; s_old += old[i]; }
+        dE = o->alpha * (double)s_new - o->alpha * (double)s_old;
+    }
+    double p = fabs(exp(-dE) * detratio);
+    *p_out = p;
+    int accept = p > 1.0 || next_uniform(o) < p;
+    if (accept) {
+        orc_reverse_build_stack(o);
+        orc_propagate(o);
+    } else {
+        memcpy(o->conf, old, (size_t)n * M);
+    }
+    free(tempvf);
+    free(old);
+    return accept;
+}
+
 EXPORT void orc_get_greens(oracle_t *o, double *out, int physical) {
     int n = o->n;
     if (!physical) { memcpy(out, o->greens, sizeof(double) * o->flv * NN); return; }

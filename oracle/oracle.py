@@ -78,6 +78,8 @@ def lib():
         L.orc_uniform_at.restype = C.c_double
         L.orc_generate_chunks.argtypes = [C.c_int, C.c_int, _ip, _ip]
         L.orc_generate_chunks.restype = C.c_int
+        L.orc_global_update.argtypes = [vp, _i8, C.POINTER(C.c_double)]
+        L.orc_global_update.restype = C.c_int
         _lib = L
     return _lib
 
@@ -223,6 +225,15 @@ class OracleChain:
         G = np.zeros((self.n, self.n * self.flv), order="F")
         lib().orc_calculate_greens_slice(self._h, slice_, G)
         return np.stack([G[:, b * self.n:(b + 1) * self.n] for b in range(self.flv)])
+
+    def global_update(self, proposal):
+        """global_update with `proposal` as the new conf (global_updates.jl:191-207): returns
+        (accepted, p)."""
+        p = C.c_double()
+        r = lib().orc_global_update(self._h, _f(proposal, np.int8), C.byref(p))
+        if r < 0:
+            raise RuntimeError("global update needs current_slice == 1 and direction == 1")
+        return bool(r), p.value
 
     def state(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -31,6 +31,29 @@ HONEYCOMB2_FROM = [1, 1, 1, 5, 5, 5, 2, 2, 2, 6, 6, 6, 3, 3, 3, 7, 7, 7, 4, 4, 4
 HONEYCOMB2_TO = [5, 6, 7, 1, 2, 3, 6, 5, 8, 2, 1, 4, 7, 8, 5, 3, 4, 1, 8, 7, 6, 4, 3, 2]
 
 
+def write_config_a_raw(T):
+    """config_a_raw.bin: the config_a.npz vectors for the C harness (tests/c_harness), raw
+    little-endian: int32 n, slices, count = 2 n slices; float64 alpha; float64 T[n*n],
+    G_start[n*n], G_end[n*n], G_slice0[n*n], G_slice13[n*n], G_slice40[n*n] (column-major);
+    int8 conf[n*slices], final_conf[n*slices]; float64 uniforms[count], p[count]; uint8
+    accepted[count]."""
+    z = np.load(os.path.join(HERE, "config_a.npz"))
+    n, M = z["conf"].shape
+    cnt = 2 * n * M
+    with open(os.path.join(HERE, "config_a_raw.bin"), "wb") as f:
+        f.write(np.array([n, M, cnt], dtype="<i4").tobytes())
+        f.write(np.array([float(z["alpha"])], dtype="<f8").tobytes())
+        for key in ("G_start", "G_end", "G_slice0", "G_slice13", "G_slice40"):
+            if key == "G_start":
+                f.write(np.asarray(T, dtype="<f8").ravel(order="F").tobytes())
+            f.write(np.asarray(z[key], dtype="<f8").ravel(order="F").tobytes())
+        for key in ("conf", "final_conf"):
+            f.write(np.asarray(z[key], dtype=np.int8).ravel(order="F").tobytes())
+        f.write(np.asarray(z["uniforms"], dtype="<f8").tobytes())
+        f.write(np.asarray(z["p"], dtype="<f8").tobytes())
+        f.write(np.asarray(z["accepted"], dtype=np.uint8).tobytes())
+
+
 def main():
     from conftest import load_pkg
     from oracle import oracle as O
@@ -62,6 +85,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "config_a.npz"), conf=field.conf, uniforms=u,
                         accepted=dec, p=p, final_conf=ch.get_field(), G_start=G_start,
                         G_end=ch.greens()[0], alpha=field.alpha, **g)
+    write_config_a_raw(model.hopping_matrix())
 
     ed = {}
     for name, U, mu in (("repulsive", -1.0, 0.0), ("attractive_mu1", 1.0, 1.0)):

@@ -163,3 +163,13 @@ def test_footprint_is_monotone(pkg):
     b = _lib.footprint_bytes(576, 1, 120, 10)
     c = _lib.footprint_bytes(256, 2, 100, 10, "mixed")
     assert 20e6 < a < 80e6 and b > a and c > a
+
+
+def test_c_harness_is_built_and_loads_the_library():
+    """The compiled C harness (tests/c_harness) links against the in-tree library and starts
+    without a GPU (usage message, exit status 2, before any library call)."""
+    exe = os.path.join(ROOT, "tests", "c_harness", "dqmc_harness")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.dirname(exe)], check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr

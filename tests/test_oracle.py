@@ -289,3 +289,64 @@ def test_oracle_dqmc_matches_ed(pkg, oracle, case):
         assert close(G[b], Ged[b * 4:(b + 1) * 4, b * 4:(b + 1) * 4])
     assert close(ms.mean("occupation"), 1 - np.diag(Ged)[: 4 * nb])
     assert close(ms.mean("double_occupancy"), ed["double_occupancy"])
+
+
+@pytest.mark.parametrize("U_ref", [4.0, -4.0])
+@pytest.mark.parametrize("move", ["flip", "spatial", "random"])
+def test_global_update_probability_is_the_dense_weight_ratio(pkg, oracle, U_ref, move):
+    """propose_global_from_conf (global_updates.jl:136-169) against brute force on 2x2,
+    beta = 1: p = exp(-dE) prod_b det(I + B_M..B_1)_new / det(..)_old, the per-flavour
+    determinants of the dense slice-matrix products (squared for the density field's one
+    unique flavour), the way test/updates.jl:186-245 pins global moves."""
+    from conftest import make_chain
+    o, model, par, field = make_chain(oracle.OracleChain, pkg, 2, U_ref, 1.0, seed=31)
+    o.set_hopping(model.hopping_matrix())
+    o.set_field(field.conf)
+    o.set_seed(5)
+    o.build_stack()
+    n, M = o.n, o.M
+    rng = np.random.default_rng(4)
+    c_old = o.get_field()
+    if move == "flip":
+        c_new = -c_old
+    elif move == "spatial":
+        c_new = c_old[rng.permutation(n), :]
+    else:
+        c_new = np.where(rng.random(c_old.shape) < 0.3, -c_old, c_old).astype(np.int8)
+    c_new = np.asfortranarray(c_new, dtype=np.int8)
+    _, _, eT2m, _ = o.get_hopping_exp()
+    a = field.alpha
+
+    def weight(c):
+        w = 1.0
+        for b in range(field.flavors):
+            sgn = 1.0 if b == 0 else -1.0
+            P = np.eye(n)
+            for l in range(M):
+                P = eT2m @ np.diag(np.exp(sgn * a * c[:, l])) @ P
+            w *= np.linalg.det(np.eye(n) + P)
+        return w
+
+    ratio = weight(c_new) / weight(c_old)
+    if field.flavors == 1:
+        ratio = ratio * ratio * np.exp(-(a * c_new.sum() - a * c_old.sum()))
+    expected = abs(ratio)
+    o.set_uniforms(np.array([0.0]))            # accept whenever p <= 1 too
+    acc, p = o.global_update(c_new)
+    assert acc
+    assert abs(p - expected) <= 1e-10 * max(expected, 1.0), (p, expected)
+    # accepted: the field is the proposal and G is the stack rebuilt from it
+    assert np.array_equal(o.get_field(), c_new)
+    fresh, *_ = make_chain(oracle.OracleChain, pkg, 2, U_ref, 1.0, seed=31)
+    fresh.set_hopping(model.hopping_matrix())
+    fresh.set_field(c_new)
+    fresh.build_stack()
+    assert np.abs(o.greens() - fresh.greens()).max() < 1e-13
+    # rejected (uniform 1): the old field is restored and G is untouched
+    G_before = o.greens()
+    o.set_uniforms(np.array([1.0]))
+    acc2, p2 = o.global_update(c_old)
+    if p2 <= 1.0:
+        assert not acc2
+        assert np.array_equal(o.get_field(), c_new)
+        assert np.array_equal(o.greens(), G_before)
