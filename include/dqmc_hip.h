@@ -255,6 +255,19 @@ int dqmc_la_calc_greens_udt(int n, const double* Ul, const double* Dl, const dou
 int dqmc_la_gemm(int n, int trans_a, int trans_b, const double* A, const double* B, double* C,
                  int precision);
 
+/* The UDT of dqmc_la_udt on device memory, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  X, U, T (n x n, column-major,
+ * ld = n), D (n) and pivot (n) are device pointers on the current device; U must not alias
+ * X or T, nor T alias X.  device_errors (device pointer, may be NULL) is incremented by
+ * broken pivot selections.  Runs asynchronously on `stream` (a hipStream_t; NULL = the null
+ * stream), with its working storage in a static device array of the library sized for
+ * n <= DQMC_NOALLOC_MAX_N (about 8.4 MB per GPU, held until the process ends); larger n is
+ * refused with DQMC_E_INVALID.  Calls that overlap on one GPU share that array and race.  */
+#define DQMC_NOALLOC_MAX_N 1024
+int dqmc_la_udt_noalloc(int n, const double* X, int apply_pivot, double* U, double* D, double* T,
+                        int32_t* pivot, uint64_t* device_errors, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

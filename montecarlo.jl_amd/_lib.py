@@ -61,6 +61,7 @@ EXPORTED = [
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
     "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
     "dqmc_server_result", "dqmc_server_status", "dqmc_server_destroy", "dqmc_server_last_error",
+    "dqmc_la_udt_noalloc",
 ]
 
 KERNEL_KINDS = ["gemm_f64", "gemm_f32", "udt", "rdivp", "local", "local_apply", "udt_panel", "udt_update"]
@@ -125,6 +126,7 @@ def load():
         "dqmc_server_result": [vp, i64, vp, i, vp, i, C.POINTER(i64), C.POINTER(Stats), vp],
         "dqmc_server_status": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i), C.POINTER(i64)],
         "dqmc_server_destroy": [vp],
+        "dqmc_la_udt_noalloc": [i, vp, i, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdqmc_hip.so")
-SOURCES = ["gemm.hip", "qr.hip", "local.hip", "util.hip", "measure.hip", "dqmc.hip", "serve.hip"]
+SOURCES = ["gemm.hip", "qr.hip", "local.hip", "util.hip", "measure.hip", "dqmc.hip", "serve.hip",
+           "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
