@@ -5,6 +5,7 @@ Follows MonteCarlo.jl's lattice code:
   flat indexing   src/lattices/lattice.jl:240-254  (site = x + Lx (y-1) + Lx Ly (basis-1), 1-based)
   bond shifting   src/lattices/lattice.jl:327-357  (_shift_Bravais, periodic wrap with mod1)
   directed bonds  src/lattices/lattice.jl:367-374  (for idx in 1:prod(Ls), for b in unitcell.bonds)
+  Bravais shifts  src/lattices/lattice_cache.jl:71-80, 226-242 (Bravais_dir2srctrg / srctrg2dir)
 
 Indices returned by this module are 0-based; `bonds(directed=True)` yields the same
 sequence as the reference's `bonds(l, Val(true))` shifted by one.
@@ -60,6 +61,36 @@ class Lattice:
             out += f * t
             f *= L
         return flat + bond.frm * f, out + bond.to * f
+
+    @property
+    def n_bravais(self) -> int:
+        return int(np.prod(self.Ls))
+
+    @property
+    def n_basis(self) -> int:
+        return len(self.unitcell.sites)
+
+    def bravais_dir2srctrg(self) -> np.ndarray:
+        """`l[:Bravais_dir2srctrg]` (lattice_cache.jl:226-242) as an (Ndir, N) int32 array,
+        0-based: row `dir` is the Bravais shift with (column-major) sub-index `dir`, entry `src`
+        the Bravais cell reached from `src` by that shift, periodically wrapped."""
+        N = self.n_bravais
+        out = np.zeros((N, N), dtype=np.int32)
+        subs = np.array(np.unravel_index(np.arange(N), self.Ls, order="F"))   # (D, N)
+        Ls = np.array(self.Ls)[:, None]
+        for shift in range(N):
+            trg = (subs + subs[:, shift:shift + 1]) % Ls
+            out[shift] = np.ravel_multi_index(tuple(trg), self.Ls, order="F")
+        return out
+
+    def bravais_srctrg2dir(self) -> np.ndarray:
+        """`l[:Bravais_srctrg2dir]` (lattice_cache.jl:71-80), 0-based (N, N)."""
+        d2st = self.bravais_dir2srctrg()
+        N = self.n_bravais
+        out = np.full((N, N), -1, dtype=np.int32)
+        for d in range(N):
+            out[np.arange(N), d2st[d]] = d
+        return out
 
     def bonds(self, directed: bool = True) -> List[Tuple[int, int]]:
         """(from, to) pairs, 0-based.  directed=True matches `bonds(l, Val(true))`."""

@@ -12,6 +12,7 @@ Restates the reference's ED harness test/ED/ED.jl:
   expectation values ED.jl:480-511 <O> = sum_k exp(-beta E_k) <k|O|k> / Z
   Greens             ED.jl:145-153, 515-536: G[(s1,i1),(s2,i2)] = <c_{i2 s2} c+_{i1 s1}>
   energy             ED.jl:539-552
+  integrated <A(tau) B(0)>  ED.jl:659-718 (trapezoid over tau = 0, dtau, .., beta)
 Dense matrices are fine: 2x2 has 4^4 = 256 states.
 """
 from __future__ import annotations
@@ -85,6 +86,36 @@ class HubbardED:
                         O = self.c[_mode(i2, s2)] @ self.cd[_mode(i1, s1)]
                         G[s1 * N + i1, s2 * N + i2] = self.expect(O)
         return G
+
+    def integrated(self, O1: np.ndarray, O2: np.ndarray, step: float) -> float:
+        """expectation_value_integrated (ED.jl:659-718): sum over tau = 0, step, .., beta of
+        trapezoid weights (step / 2 at the ends) times <O1(tau) O2(0)>, with
+        <O1(tau) O2> = sum_nm exp(-(beta - tau) E_n - tau E_m) (O1)_nm (O2)_mn / Z."""
+        V, E = self.V, self.E - self.E.min()
+        X = (V.T @ O1 @ V) * (V.T @ O2 @ V).T
+        Z = np.exp(-self.beta * E).sum()
+        M = int(round(self.beta / step))
+        out = 0.0
+        for i in range(M + 1):
+            tau = step * i
+            f = np.exp(-(self.beta - tau) * E)[:, None] * np.exp(-tau * E)[None, :]
+            out += (0.5 * step if i in (0, M) else step) * float((f * X).sum()) / Z
+        return out
+
+    def number(self, site: int) -> np.ndarray:
+        """number_operator(site) (ED.jl:722): n_up + n_dn."""
+        return self.n[site][0] + self.n[site][1]
+
+    def spin(self, site: int, direction: str) -> tuple:
+        """m_x, m_y, m_z of `site` (ED.jl:209-244) as (real matrix A, factor f) with m = f A:
+        m_x = c+_up c_dn + c+_dn c_up, m_y = -i (c+_up c_dn - c+_dn c_up), m_z = n_up - n_dn."""
+        up, dn = _mode(site, 0), _mode(site, 1)
+        X = self.cd[up] @ self.c[dn]
+        if direction == "x":
+            return X + X.T, 1.0
+        if direction == "y":
+            return X - X.T, -1j
+        return self.n[site][0] - self.n[site][1], 1.0
 
     def occupation(self) -> np.ndarray:
         """<n_{i,s}>, spin-up sites first."""
