@@ -207,6 +207,49 @@ int dqmc_set_greens(dqmc_chain_t h, const double* G, int ld);
  * the chain's current G.                                                               */
 int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* G, int ld);
 
+/* Time-displaced Green's function G(k <- l) = <c_i(k dtau) c_j^dagger(l dtau)>, 0 <= k, l <=
+ * slices: greens(mc, k, l) (physical = 1: e^{dtau T/2} G e^{-dtau T/2}, greens.jl:114-125)
+ * or calculate_greens(mc, k, l) (physical = 0), unequal_time_stack.jl:297-338, from the
+ * chain's unequal-time stack (forward, backward and inverse UDT slots per safe_mult range,
+ * :1-42, 124-278, built lazily on first use for the current field and kept until the field
+ * or the hopping changes); k >= l by calculate_greens_full1! (:535-617), k < l by
+ * calculate_greens_full2! (:620-697).  n x (flavors*n).  Leaves the chain's G, stack and
+ * state untouched.                                                                     */
+int dqmc_unequal_time_greens(dqmc_chain_t h, int k, int l, double* G, int ld, int physical);
+
+/* CombinedGreensIterator(mc; start, stop, recalculate) iterated to its end, which is also
+ * TimeIntegral's iteration (measurements/greens_iterators.jl:83-407): for l = start..stop
+ * the physical G(0, l), G(l, 0), G(l, l), written to `out` as 3 (stop - start + 1)
+ * consecutive n x (flavors*n) matrices (leading dimension ld) in that order per l.  G(0, 0)
+ * is the chain's own G when its current slice is 1 (FP64 form), else full1!(0, 0); every
+ * `recalculate` slices (the reference's default is 2 safe_mult) the triple is recomputed
+ * from the unequal-time stack, in between advanced by one slice product per step and
+ * re-decomposed every safe_mult slices.  out_on_device = 1: `out` is device memory on the
+ * chain's GPU and ld must be n.  Leaves the chain's G, stack and state untouched.       */
+int dqmc_combined_greens(dqmc_chain_t h, int start, int stop, int recalculate, double* out, int ld,
+                         int out_on_device);
+
+/* charge_density_susceptibility(mc, model) and spin_density_susceptibility(mc, model, dir)
+ * measured once for the current field (measurements/constructors/charge_density.jl:47,
+ * spin_density.jl:54): the TimeIntegral iteration over l = 0..slices (trapezoid weights dtau,
+ * halved at both ends; measurements/generic.jl:337-370) with G00 = the chain's current
+ * physical G, summed by EachSitePairByDistance (generic.jl:434-460) with the observable's
+ * full kernel (reduced = 0) or reduced kernel (reduced = 1): full/reduced_cdc_kernel
+ * (charge_density.jl:62-170) over the flavour pairs, full/reduced_sdc_{x,y,z}_kernel
+ * (spin_density.jl:68-330); one flavour block is the reference's DiagonallyRepeatingMatrix
+ * form (with total_flavors), two blocks its BlockDiagonal form.  The sum is divided by the
+ * number of sites (finish!, generic.jl:574-586).  dir2srctrg: the lattice's
+ * Bravais_dir2srctrg (lattice_cache.jl:226-242), 0-based, n_bravais rows (row = shift) of
+ * n_bravais targets; sites are numbered Bravais cell + n_bravais * basis.  out: n_bravais *
+ * n_basis^2 values, index dir + n_bravais (b1 + n_basis b2).  Allocates its accumulator per
+ * call.                                                                                  */
+#define DQMC_CHARGE_DENSITY 0
+#define DQMC_SPIN_DENSITY_X 1
+#define DQMC_SPIN_DENSITY_Y 2
+#define DQMC_SPIN_DENSITY_Z 3
+int dqmc_density_susceptibility(dqmc_chain_t h, int observable, int recalculate, const int32_t* dir2srctrg,
+                                int n_bravais, int n_basis, int total_flavors, int reduced, double* out);
+
 /* global_update(mc, model, field) with `proposal` (n x slices, +-1) as the proposed conf
  * (global_updates.jl:191-207; propose_global_from_conf :136-169, inv_det :64-121,
  * calculate_inv_greens_udt :25-47, accept_global! :171-188): p = |exp(-dE) detratio| from
@@ -224,6 +267,12 @@ int dqmc_get_stats(dqmc_chain_t h, dqmc_stats_t* stats);
  * the event is a device error.  No reference counterpart (the reference's update is
  * sequential host code).                                                               */
 int dqmc_set_wait_limit(dqmc_chain_t h, int64_t polls);
+/* A sweep's launch sequence is the same every sweep (the decisions are made on the device),
+ * so FP64 chains capture one sweep as a HIP graph on first use and replay it while the host
+ * state it baked in (propagate state, buffer roles, seed / uniform / log buffers, wait bound)
+ * is unchanged, recapturing otherwise.  enable = 0 launches kernel by kernel.  Results are
+ * the same either way.  No reference counterpart (the reference's sweep is host code). */
+int dqmc_set_sweep_graph(dqmc_chain_t h, int enable);
 /* max|G_wrapped - G_scratch| at every stabilisation since the last call (mixed precision:
  * FP32-propagated G against the FP64 recompute).                                       */
 int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, int64_t* length);

@@ -55,8 +55,9 @@ EXPORTED = [
     "dqmc_set_decision_log", "dqmc_get_decision_log", "dqmc_build_stack",
     "dqmc_build_stack_forward", "dqmc_reverse_build_stack", "dqmc_propagate",
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
-    "dqmc_calculate_greens", "dqmc_global_update", "dqmc_get_state", "dqmc_get_stats",
-    "dqmc_set_wait_limit",
+    "dqmc_calculate_greens", "dqmc_unequal_time_greens", "dqmc_combined_greens",
+    "dqmc_density_susceptibility", "dqmc_global_update", "dqmc_get_state", "dqmc_get_stats",
+    "dqmc_set_wait_limit", "dqmc_set_sweep_graph",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
     "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
@@ -107,10 +108,14 @@ def load():
         "dqmc_get_greens": [vp, _dp, i, i],
         "dqmc_set_greens": [vp, _dp, i],
         "dqmc_calculate_greens": [vp, i, _dp, i],
+        "dqmc_unequal_time_greens": [vp, i, i, _dp, i, i],
+        "dqmc_combined_greens": [vp, i, i, i, _dp, i, i],
+        "dqmc_density_susceptibility": [vp, i, i, _i32, i, i, i, i, _dp],
         "dqmc_global_update": [vp, _i8, i, C.POINTER(i), C.POINTER(d)],
         "dqmc_get_state": [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)],
         "dqmc_get_stats": [vp, C.POINTER(Stats)],
         "dqmc_set_wait_limit": [vp, i64],
+        "dqmc_set_sweep_graph": [vp, i],
         "dqmc_get_stabilization_log": [vp, vp, i64, C.POINTER(i64)],
         "dqmc_synchronize": [vp],
         "dqmc_la_udt": [i, _dp, i, _dp, _dp, _dp, _i32],
@@ -385,6 +390,43 @@ class HIPChain:
         self._c(load().dqmc_calculate_greens(self._h, slice_, G, self.n), "dqmc_calculate_greens")
         return self._blocks(G)
 
+    def unequal_time_greens(self, k: int, l: int, physical: bool = True) -> np.ndarray:
+        """greens(mc, k, l) (physical) or calculate_greens(mc, k, l): G(k <- l), (flv, n, n)."""
+        G = np.zeros((self.n, self.n * self.flv), order="F")
+        self._c(load().dqmc_unequal_time_greens(self._h, k, l, G, self.n, int(physical)),
+                "dqmc_unequal_time_greens")
+        return self._blocks(G)
+
+    def combined_greens(self, start: int = 0, stop: int | None = None,
+                        recalculate: int | None = None):
+        """CombinedGreensIterator(mc; start, stop, recalculate) iterated to its end: a list of
+        (G(0, l), G(l, 0), G(l, l)) for l = start..stop, each (flv, n, n), physical forms."""
+        stop = self.M if stop is None else stop
+        rec = 2 * self.safe_mult if recalculate is None else recalculate
+        cnt = max(0, stop - start + 1)
+        n, f = self.n, self.flv
+        out = np.zeros((n, f * n, 3 * cnt), order="F")
+        self._c(load().dqmc_combined_greens(self._h, start, stop, rec, out, n, 0),
+                "dqmc_combined_greens")
+        return [tuple(self._blocks(np.asfortranarray(out[:, :, 3 * i + j])) for j in range(3))
+                for i in range(cnt)]
+
+    def density_susceptibility(self, lattice, observable: str = "charge",
+                               recalculate: int | None = None, reduced: bool = False,
+                               total_flavors: int = 2) -> np.ndarray:
+        """charge_density_susceptibility (observable "charge") or
+        spin_density_susceptibility (observable "x", "y" or "z") for the current field:
+        (Ndir, B, B) with Ndir = Bravais cells of `lattice`, B = its basis sites."""
+        code = {"charge": 0, "x": 1, "y": 2, "z": 3}[observable]
+        N, B = lattice.n_bravais, lattice.n_basis
+        d2st = np.ascontiguousarray(lattice.bravais_dir2srctrg(), dtype=np.int32)
+        rec = 2 * self.safe_mult if recalculate is None else recalculate
+        out = np.zeros(N * B * B, order="F")
+        self._c(load().dqmc_density_susceptibility(self._h, code, rec, d2st, N, B, total_flavors,
+                                                   int(reduced), out),
+                "dqmc_density_susceptibility")
+        return out.reshape((N, B, B), order="F")
+
     def global_update(self, proposal):
         """global_update with `proposal` as the new conf (dqmc_global_update): (accepted, p)."""
         c = _f(proposal, np.int8)
@@ -416,6 +458,10 @@ class HIPChain:
     def set_wait_limit(self, polls: int):
         """Bound of the window kernel's internal wait (dqmc_set_wait_limit)."""
         self._c(load().dqmc_set_wait_limit(self._h, polls), "dqmc_set_wait_limit")
+
+    def set_sweep_graph(self, enable: bool):
+        """Replay sweeps from a captured HIP graph (default) or launch kernel by kernel."""
+        self._c(load().dqmc_set_sweep_graph(self._h, int(enable)), "dqmc_set_sweep_graph")
 
     def stabilization_log(self) -> np.ndarray:
         length = C.c_int64()

@@ -9,13 +9,23 @@ namespace dqmc {
 
 // A diagonal operand d[k] given either as a vector, or generated from a Hirsch
 // field column: d[k] = conf[k] > 0 ? pos : neg  (exp(+-alpha) tables, so the
-// interaction matrix exp(power * alpha * x) never needs its own buffer).
+// interaction matrix exp(power * alpha * x) never needs its own buffer).  A vector can be
+// read through the reference's elementwise maps of D vectors (linalg/real.jl:143-184):
+enum DiagMode {
+  kDiagPlain = 0,    // v
+  kDiagInv = 1,      // 1 / v                 (vinv!)
+  kDiagMaxInv = 2,   // 1 / max(1, v)         (vmaxinv!)
+  kDiagMin = 3,      // min(1, v)             (vmin!)
+};
 struct DiagSpec {
   const double* vec;     // nullptr: not a vector
   const int8_t* conf;    // nullptr: not a field column
   double pos, neg;       // values for conf > 0 / conf < 0
-  int invert;            // use 1 / vec[k]
+  int mode;              // DiagMode applied to vec[k]
 };
+__host__ __device__ inline double diag_map(double v, int mode) {
+  return mode == kDiagPlain ? v : mode == kDiagInv ? 1.0 / v : mode == kDiagMaxInv ? 1.0 / fmax(1.0, v) : fmin(1.0, v);
+}
 
 // Row held in accumulator element r of lane `lane` for the 16x16x4 MFMA tiles
 // (column = lane % 16 for both):
@@ -26,15 +36,15 @@ template <> __device__ __forceinline__ int mfma_row<double>(int lane, int r) { r
 template <> __device__ __forceinline__ int mfma_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
 
 __host__ __device__ inline DiagSpec diag_none() { return DiagSpec{nullptr, nullptr, 1.0, 1.0, 0}; }
-__host__ __device__ inline DiagSpec diag_vec(const double* v, int inv = 0) {
-  return DiagSpec{v, nullptr, 1.0, 1.0, inv};
+__host__ __device__ inline DiagSpec diag_vec(const double* v, int mode = kDiagPlain) {
+  return DiagSpec{v, nullptr, 1.0, 1.0, mode};
 }
 __host__ __device__ inline DiagSpec diag_field(const int8_t* c, double pos, double neg) {
   return DiagSpec{nullptr, c, pos, neg, 0};
 }
 __host__ __device__ inline bool diag_active(const DiagSpec& d) { return d.vec || d.conf; }
 __device__ inline double diag_at(const DiagSpec& d, int k) {
-  if (d.vec) return d.invert ? 1.0 / d.vec[k] : d.vec[k];
+  if (d.vec) return diag_map(d.vec[k], d.mode);
   if (d.conf) return d.conf[k] > 0 ? d.pos : d.neg;
   return 1.0;
 }
@@ -149,6 +159,13 @@ hipError_t launch_f64_to_f32(float* dst, const double* src, int64_t count, hipSt
 hipError_t launch_f32_to_f64(double* dst, const float* src, int64_t count, hipStream_t s);
 hipError_t launch_symmetrize(double* A, int n, hipStream_t s);
 // out[slot] = max |a - b| over count elements (a may be f32 when a32 != nullptr)
+// One time slice of a density susceptibility (measure.hip): acc[dir + N (b1 + B b2)] += w sum_src K
+// over the N x B x B outputs; d2st is the Bravais shift table (N x N, row dir).
+enum DensityObservable { kDensityCharge = 0, kDensitySpinX = 1, kDensitySpinY = 2, kDensitySpinZ = 3 };
+hipError_t launch_density_slice(const double* G00, const double* G0l, const double* Gl0, const double* Gll,
+                                const int* d2st, int N, int B, int n, int flv, double total_flavors, int observable,
+                                int first_slice, int reduced, double w, double* acc, hipStream_t s);
+
 // Equal-time observables from the physical G (measure.hip); out holds 5 + (flv + 1) n values.
 hipError_t launch_measure(const double* G, const double* T, int n, int flv, double U, double* out,
                           hipStream_t s);
@@ -160,6 +177,13 @@ hipError_t launch_norm1(const double* A, int n, double* out, hipStream_t s);
 // out = in * diag(d) for nmat consecutive n x n matrices (d: nmat consecutive vectors)
 hipError_t launch_scale_cols(double* out, const double* in, const double* d, int n, int nmat,
                              hipStream_t s);
+
+// out_b = diag(map(dl_b)) in_b diag(map(dr_b)) [+ add_b] and A_b = diag(map(v_b)) for nmat
+// consecutive n x n blocks, vectors b n apart (DiagMode maps; a null vector means no scaling,
+// a null add no addend)
+hipError_t launch_diag_scale(double* out, const double* in, const double* dl, int lmode, const double* dr,
+                             int rmode, const double* add, int n, int nmat, hipStream_t s);
+hipError_t launch_diag_matrix(double* A, const double* v, int mode, int n, int nmat, hipStream_t s);
 
 // Many-chain elementwise launches: one item per (chain, buffer), in device memory.
 enum ElemOp { kElemCopy = 0, kElemIdentity = 1, kElemFill = 2 };
@@ -232,11 +256,11 @@ struct WindowArgs {
 };
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
-// Many-chain window launch: `count` chains' WindowArgs in device memory (same flv and W for
-// all).  Workgroup (0, c) runs chain c's window when its do_window is set; workgroups
-// (1 + t, c) apply chain c's previous window to tile t when its Yp is set.  max_n sizes the
-// grid; every item carries its own n.
-hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int flv, int W, int max_n,
+// Many-chain window launch: `count` chains' WindowArgs in device memory (the same W for all,
+// each with its own n and flavour count).  Workgroup (0, c) runs chain c's window when its
+// do_window is set; workgroups (1 + t, c) apply chain c's previous window to tile t when its
+// Yp is set.  max_flv and max_n size the grid.
+hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int max_flv, int W, int max_n,
                                    hipStream_t s);
 int window_size(int n, int flv);
 

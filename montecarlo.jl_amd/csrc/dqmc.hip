@@ -123,6 +123,37 @@ struct dqmc_chain_s {
   std::vector<WindowArgs<double>> rwin;
   long long steps = 0;               // slice visits (sweep_spatial + propagate) in ensembles
   bool dg_valid = false;             // Dr holds the D of the last calculate_greens of G itself
+  long long fver = 0;                // bumped whenever the field or the hopping may change
+
+  // pinned host mirrors of the counters and of the stabilisation slots: the end of a sweep is
+  // two stream-ordered copies and one wait
+  ChainCounters* hctr = nullptr;
+  double* herr = nullptr;
+  bool acc_valid = false;            // acc_last is the device's accepted count
+  long long acc_last = 0;
+
+  // one sweep (2 M slice visits) as a HIP graph: captured on first use and replayed while
+  // every launch argument it baked in (state, buffer roles, streams of randomness, logs) is
+  // unchanged; FP64 chains only
+  bool use_graph = true;
+  hipGraph_t sg = nullptr;
+  hipGraphExec_t sgx = nullptr;
+  std::vector<uint64_t> sg_key;
+  int sg_cs = 0, sg_dir = 0, sg_cr = 0;
+  double *sg_G = nullptr, *sg_Gpp = nullptr;
+  long long sg_nerr = 0;
+
+  // unequal-time stack (UnequalTimeStack, unequal_time_stack.jl:1-42), one allocation made on
+  // first use: forward and backward UDT slots 1..K+1, inverse slots 1..K (flv blocks each);
+  // uU/uD/uT the inverse block of the last G(k <- l), uG its effective G
+  double *fu = nullptr, *fd = nullptr, *ft = nullptr, *bu = nullptr, *bd = nullptr, *bt = nullptr;
+  double *iu = nullptr, *id = nullptr, *it = nullptr;
+  double *uU = nullptr, *uD = nullptr, *uT = nullptr, *uG = nullptr, *uDsave = nullptr;
+  double *itm = nullptr, *itv = nullptr;  // greens iterator: 14 matrices, 3 vectors (flv blocks)
+  int ut_fwd = 1, ut_bwd = 0;        // forward slots 1..ut_fwd, backward slots ut_bwd..K+1 built
+  std::vector<char> ut_inv;          // inverse slot built
+  long long ut_ver = -1;             // fver the slots belong to
+  int ut_k = -1, ut_l = -1;          // (k, l) of uG
 
   // optional per-kernel timing with HIP events on the chain's stream
   bool prof = false;
@@ -497,35 +528,58 @@ int mul_blocks(Chain* h, const Blocks& out, const Blocks& t1, const Blocks& t2) 
   return run(h, g);
 }
 
-// add_slice_sequence_left(mc, idx), stack.jl:377-393
-int add_slice_sequence_left(Chain* h, int idx) {
+// A stack of UDT slots (s = 1, 2, ...; flv blocks per slot): the chain's own stack or one of
+// the unequal-time stacks.
+struct Slots {
+  double *u, *d, *t;
+};
+Slots chain_slots(Chain* h) { return Slots{h->us, h->ds, h->ts}; }
+Blocks SU(Chain* h, const Slots& S, int s) {
+  double* p = S.u + (size_t)(s - 1) * h->flv * h->nn;
+  return Blocks{{p, h->flv > 1 ? p + h->nn : nullptr}};
+}
+Blocks ST(Chain* h, const Slots& S, int s) {
+  double* p = S.t + (size_t)(s - 1) * h->flv * h->nn;
+  return Blocks{{p, h->flv > 1 ? p + h->nn : nullptr}};
+}
+Blocks SD(Chain* h, const Slots& S, int s) {
+  double* p = S.d + (size_t)(s - 1) * h->flv * h->n;
+  return Blocks{{p, h->flv > 1 ? p + h->n : nullptr}};
+}
+
+// add_slice_sequence_left(mc, idx), stack.jl:377-393 (and the forward build of the
+// unequal-time stack, unequal_time_stack.jl:126-138, on its slots)
+int add_slice_sequence_left(Chain* h, int idx, const Slots& S) {
   const int r0 = h->rs[idx - 1], r1 = h->re[idx - 1];
-  Blocks in = Us(h, idx);
-  Blocks dsl = Ds(h, idx);
+  Blocks in = SU(h, S, idx);
+  Blocks dsl = SD(h, S, idx);
   for (int s = r0; s <= r1; s++) {
     const bool last = s == r1;
     Blocks out = last ? B(h, h->tmp1) : ((s - r0) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
     CHK(slice_left(h, s, in.get(), out.get(), last ? dsl.get() : nullptr));
     in = out;
   }
-  CHK(udt(h, B(h, h->tmp1).get(), Us(h, idx + 1).get(), Ds(h, idx + 1).get(), B(h, h->tmp2).get(), 1));
-  return mul_blocks(h, Ts(h, idx + 1), B(h, h->tmp2), Ts(h, idx));
+  CHK(udt(h, B(h, h->tmp1).get(), SU(h, S, idx + 1).get(), SD(h, S, idx + 1).get(), B(h, h->tmp2).get(), 1));
+  return mul_blocks(h, ST(h, S, idx + 1), B(h, h->tmp2), ST(h, S, idx));
 }
+int add_slice_sequence_left(Chain* h, int idx) { return add_slice_sequence_left(h, idx, chain_slots(h)); }
 
-// add_slice_sequence_right(mc, idx), stack.jl:402-416
-int add_slice_sequence_right(Chain* h, int idx) {
+// add_slice_sequence_right(mc, idx), stack.jl:402-416 (and the backward build of the
+// unequal-time stack, unequal_time_stack.jl:141-155)
+int add_slice_sequence_right(Chain* h, int idx, const Slots& S) {
   const int r0 = h->rs[idx - 1], r1 = h->re[idx - 1];
-  Blocks in = Us(h, idx + 1);
-  Blocks dsl = Ds(h, idx + 1);
+  Blocks in = SU(h, S, idx + 1);
+  Blocks dsl = SD(h, S, idx + 1);
   for (int s = r1; s >= r0; s--) {
     const bool last = s == r0;
     Blocks out = last ? B(h, h->tmp1) : ((r1 - s) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
     CHK(slice_dagger_left(h, s, in.get(), out.get(), last ? dsl.get() : nullptr));
     in = out;
   }
-  CHK(udt(h, B(h, h->tmp1).get(), Us(h, idx).get(), Ds(h, idx).get(), B(h, h->tmp2).get(), 1));
-  return mul_blocks(h, Ts(h, idx), B(h, h->tmp2), Ts(h, idx + 1));
+  CHK(udt(h, B(h, h->tmp1).get(), SU(h, S, idx).get(), SD(h, S, idx).get(), B(h, h->tmp2).get(), 1));
+  return mul_blocks(h, ST(h, S, idx), B(h, h->tmp2), ST(h, S, idx + 1));
 }
+int add_slice_sequence_right(Chain* h, int idx) { return add_slice_sequence_right(h, idx, chain_slots(h)); }
 
 // calculate_greens_AVX!(Ul, Dl, Tl, Ur, Dr, Tr, G), stack.jl:442-496
 int calculate_greens(Chain* h, double* out) {
@@ -563,7 +617,7 @@ int calculate_greens(Chain* h, double* out) {
     auto g = batch<double>(0, 1);
     for (int b = 0; b < h->flv; b++) {
       auto it = item<double>(Ur.p[b], Tr.p[b], G.p[b], n);
-      it.dmid = diag_vec(Dr.p[b], 1);
+      it.dmid = diag_vec(Dr.p[b], kDiagInv);
       g.it[g.count++] = it;
     }
     CHK(run(h, g));
@@ -641,15 +695,34 @@ int check_device_errors(Chain* h) {
 }
 
 int harvest(Chain* h) {
+  if (h->hctr && h->ctr) {
+    HIPCHK(h, hipMemcpyAsync(h->hctr, h->ctr, sizeof(ChainCounters), hipMemcpyDeviceToHost, h->st));
+    if (h->nerr > 0)
+      HIPCHK(h, hipMemcpyAsync(h->herr, h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost, h->st));
+  }
   HIPCHK(h, hipStreamSynchronize(h->st));
   collect_times(h);
-  if (h->nerr > 0) {
-    std::vector<double> v(h->nerr);
-    HIPCHK(h, hipMemcpy(v.data(), h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost));
-    for (double x : v) absorb_error(h, x);
+  if (h->hctr && h->ctr) {
+    for (long long i = 0; i < h->nerr; i++) absorb_error(h, h->herr[i]);
     h->nerr = 0;
+    h->acc_last = h->hctr->accepted;
+    h->acc_valid = true;
+    const long long e = h->hctr->device_errors;
+    if (e > h->dev_err_seen) {
+      h->err = "device error: " + std::to_string(e - h->dev_err_seen) +
+               " kernel(s) gave up a bounded wait or met a broken pivot selection; G is not valid";
+      h->dev_err_seen = e;
+      return DQMC_E_DEVICE;
+    }
+  } else {
+    if (h->nerr > 0) {
+      std::vector<double> v(h->nerr);
+      HIPCHK(h, hipMemcpy(v.data(), h->errbuf, sizeof(double) * h->nerr, hipMemcpyDeviceToHost));
+      for (double x : v) absorb_error(h, x);
+      h->nerr = 0;
+    }
+    CHK(check_device_errors(h));
   }
-  CHK(check_device_errors(h));
   if (h->prec_viol > h->prec_viol_seen) {
     char buf[256];
     std::snprintf(buf, sizeof buf,
@@ -806,6 +879,8 @@ int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
 }
 
 int sweep_spatial(Chain* h) {
+  h->fver++;
+  h->acc_valid = false;
   if (h->cs < 1 || h->cs > h->M) {
     h->err = "sweep_spatial: current slice outside 1..slices (build the stack first)";
     return DQMC_E_STATE;
@@ -849,6 +924,7 @@ int expm_device(Chain* h, double c, double norm1_T, double* out) {
 }
 
 int finish_hopping(Chain* h) {
+  h->fver++;
   if (h->prec == DQMC_PRECISION_MIXED) {
     HIPCHK(h, launch_f64_to_f32(h->eT2m32, h->eT2m, (int64_t)h->nn, h->st));
     HIPCHK(h, launch_f64_to_f32(h->eT2p32, h->eT2p, (int64_t)h->nn, h->st));
@@ -921,6 +997,438 @@ int greens_from_scratch(Chain* h, int slice, double* out) {
     CHK(chain_step(h->Ul, h->Dl, h->Tl, true));
   }
   return calculate_greens(h, out);
+}
+
+// ---- unequal-time Green's functions (unequal_time_stack.jl) --------------------------------
+Slots ut_forward(Chain* h) { return Slots{h->fu, h->fd, h->ft}; }
+Slots ut_backward(Chain* h) { return Slots{h->bu, h->bd, h->bt}; }
+Slots ut_inverse(Chain* h) { return Slots{h->iu, h->id, h->it}; }
+
+// initialize_stack(mc, ::UnequalTimeStack), unequal_time_stack.jl:64-113: one allocation for
+// the three stacks and the block / result buffers, made on first use.
+int ut_alloc(Chain* h) {
+  if (h->fu) return DQMC_OK;
+  const size_t K = (size_t)h->K, f = (size_t)h->flv, nn = h->nn, n = (size_t)h->n;
+  const size_t mats = f * nn, vecs = f * n;
+  auto up = [](size_t x) { return (x + 31) & ~(size_t)31; };  // 256-byte aligned offsets
+  const size_t total = up(2 * (K + 1) * mats) * 2 + up((K + 1) * vecs) * 2 + up(K * mats) * 2 + up(K * vecs) +
+                       up(mats) * 3 + up(vecs) * 2 + up(14 * mats) + up(3 * vecs);
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, total * sizeof(double));
+  if (e != hipSuccess) {
+    h->err = std::string("unequal-time stack: hipMalloc: ") + hipGetErrorString(e);
+    return DQMC_E_ALLOC;
+  }
+  h->allocs.push_back(q);
+  double* p = static_cast<double*>(q);
+  auto take = [&](size_t count) { double* r = p; p += up(count); return r; };
+  h->fu = take((K + 1) * mats); h->ft = take((K + 1) * mats);
+  h->bu = take((K + 1) * mats); h->bt = take((K + 1) * mats);
+  h->fd = take((K + 1) * vecs); h->bd = take((K + 1) * vecs);
+  h->iu = take(K * mats); h->it = take(K * mats); h->id = take(K * vecs);
+  h->uU = take(mats); h->uT = take(mats); h->uG = take(mats);
+  h->uD = take(vecs); h->uDsave = take(vecs);
+  h->itm = take(14 * mats); h->itv = take(3 * vecs);
+  h->ut_ver = -1;
+  return DQMC_OK;
+}
+
+// The stacks hold the current field: after any change of field or hopping the built slots
+// are dropped (the last_update test of lazy_build_*, unequal_time_stack.jl:190-196).
+int ut_validate(Chain* h) {
+  CHK(ut_alloc(h));
+  if (h->ut_ver == h->fver) return DQMC_OK;
+  h->ut_ver = h->fver;
+  h->ut_fwd = 1;
+  h->ut_bwd = h->K + 1;
+  h->ut_inv.assign(h->K, 0);
+  h->ut_k = h->ut_l = -1;
+  const Slots F = ut_forward(h), Bk = ut_backward(h);
+  CHK(udt_identity(h, SU(h, F, 1).p[0], SD(h, F, 1).p[0], ST(h, F, 1).p[0]));
+  return udt_identity(h, SU(h, Bk, h->K + 1).p[0], SD(h, Bk, h->K + 1).p[0], ST(h, Bk, h->K + 1).p[0]);
+}
+
+// lazy_build_forward!(mc, s, upto), unequal_time_stack.jl:183-220
+int ut_build_forward(Chain* h, int upto) {
+  for (int idx = h->ut_fwd; idx <= upto - 1; idx++) CHK(add_slice_sequence_left(h, idx, ut_forward(h)));
+  h->ut_fwd = std::max(upto, h->ut_fwd);
+  return DQMC_OK;
+}
+
+// lazy_build_backward!(mc, s, downto), unequal_time_stack.jl:222-250
+int ut_build_backward(Chain* h, int downto) {
+  for (int idx = h->ut_bwd - 1; idx >= downto; idx--) CHK(add_slice_sequence_right(h, idx, ut_backward(h)));
+  h->ut_bwd = std::min(downto, h->ut_bwd);
+  return DQMC_OK;
+}
+
+// M_b <- eV(slice)^-1 eT2^-1 M_b              multiply_slice_matrix_inv_left!  (stack.jl:349-357)
+int slice_inv_left(Chain* h, int slice, double* const in[], double* const out[], const double* const dr[]) {
+  auto g = batch<double>(0, 0);
+  for (int b = 0; b < h->flv; b++) {
+    auto it = item<double>(h->eT2p, in[b], out[b], h->n);
+    it.dl = ev(h, b, slice, -1.0);
+    if (dr) it.dr = diag_vec(dr[b]);
+    g.it[g.count++] = it;
+  }
+  return run(h, g);
+}
+// M_b <- M_b eV(slice)^-1 eT2^-1              multiply_slice_matrix_inv_right! (stack.jl:339-347)
+int slice_inv_right(Chain* h, int slice, double* const in[], double* const out[]) {
+  auto g = batch<double>(0, 0);
+  for (int b = 0; b < h->flv; b++) {
+    auto it = item<double>(in[b], h->eT2p, out[b], h->n);
+    it.dmid = ev(h, b, slice, -1.0);
+    g.it[g.count++] = it;
+  }
+  return run(h, g);
+}
+
+// lazy_build_inv!(mc, s, from, to), unequal_time_stack.jl:252-278: inverse slot idx holds
+// B_{r0}^-1 .. B_{r1}^-1 of range idx as a pivoted UDT.
+int ut_build_inverse(Chain* h, int from, int to) {
+  const Slots I = ut_inverse(h);
+  for (int idx = from; idx <= to; idx++) {
+    if (h->ut_inv[idx - 1]) continue;
+    h->ut_inv[idx - 1] = 1;
+    const int r0 = h->rs[idx - 1], r1 = h->re[idx - 1];
+    CHK(identity(h, h->cur, h->flv));
+    Blocks in = B(h, h->cur);
+    for (int sl = r1; sl >= r0; sl--) {
+      Blocks out = sl == r0 ? B(h, h->tmp1) : ((r1 - sl) % 2 == 0 ? B(h, h->tmp2) : B(h, h->cur));
+      CHK(slice_inv_left(h, sl, in.get(), out.get(), nullptr));
+      in = out;
+    }
+    CHK(udt(h, B(h, h->tmp1).get(), SU(h, I, idx).get(), SD(h, I, idx).get(), ST(h, I, idx).get(), 1));
+  }
+  return DQMC_OK;
+}
+
+// _find_range_with_value(mc, val), unequal_time_stack.jl:338-367: the 1-based range holding
+// val; 0 below all ranges, K + 1 above.
+int find_range(const Chain* h, int val) {
+  if (val < 1) return 0;
+  if (val > h->re[h->K - 1]) return h->K + 1;
+  for (int i = 0; i < h->K; i++)
+    if (h->rs[i] <= val && val <= h->re[i]) return i + 1;
+  return h->K + 1;
+}
+
+// compute_inverse_udt_block!(mc, s, low, high), unequal_time_stack.jl:383-459:
+// uU diag(uD) uT = B_{low+1}^-1 .. B_high^-1
+int ut_inverse_block(Chain* h, int low, int high) {
+  const int lower = find_range(h, low) + 1;
+  const int upper = find_range(h, high + 1) - 1;
+  CHK(ut_build_inverse(h, lower, upper));
+  const Slots I = ut_inverse(h);
+  CHK(udt_identity(h, h->uU, h->uD, h->uT));
+  const int n = h->n;
+  for (int idx = lower; idx <= upper; idx++) {
+    {  // tmp1 = D (T inv_u) inv_d
+      auto g = batch<double>(0, 0);
+      Blocks iu = SU(h, I, idx), idd = SD(h, I, idx);
+      for (int b = 0; b < h->flv; b++) {
+        auto it = item<double>(blk(h->uT, h, b), iu.p[b], blk(h->tmp1, h, b), n);
+        it.dl = diag_vec(h->uD + (size_t)b * n);
+        it.dr = diag_vec(idd.p[b]);
+        g.it[g.count++] = it;
+      }
+      CHK(run(h, g));
+    }
+    CHK(udt(h, B(h, h->tmp1).get(), B(h, h->tmp2).get(), Bd(h, h->uD).get(), B(h, h->cur).get(), 1));
+    CHK(mul_blocks(h, B(h, h->uT), B(h, h->cur), ST(h, I, idx)));   // T = T' inv_t
+    CHK(mul_blocks(h, B(h, h->tmp1), B(h, h->uU), B(h, h->tmp2)));  // U = U U'
+    CHK(copyd(h, h->uU, h->tmp1, (size_t)h->flv * h->nn));
+  }
+  const int lower_slice = lower <= h->K ? h->rs[lower - 1] : h->re[h->K - 1] + 1;
+  const int upper_slice = upper > 0 ? h->re[upper - 1] : 0;
+  const int top = std::min(lower_slice - 1, high);
+  if (top >= low + 1) {
+    // U <- B_{low+1}^-1 .. B_top^-1 U, the last product scaled by D; then U D T <- UDT(U D) T
+    Blocks in = B(h, h->uU);
+    for (int sl = top; sl >= low + 1; sl--) {
+      const bool last = sl == low + 1;
+      Blocks out = last ? B(h, h->tmp1) : ((top - sl) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
+      CHK(slice_inv_left(h, sl, in.get(), out.get(), last ? Bd(h, h->uD).get() : nullptr));
+      in = out;
+    }
+    CHK(udt(h, B(h, h->tmp1).get(), B(h, h->uU).get(), Bd(h, h->uD).get(), B(h, h->cur).get(), 1));
+    CHK(mul_blocks(h, B(h, h->tmp2), B(h, h->cur), B(h, h->uT)));
+    CHK(copyd(h, h->uT, h->tmp2, (size_t)h->flv * h->nn));
+  }
+  const int first = std::max(upper_slice + 1, top + 1);
+  if (first <= high) {
+    Blocks in = B(h, h->uT);
+    for (int sl = first; sl <= high; sl++) {
+      const bool last = sl == high;
+      Blocks out = last ? B(h, h->uT) : ((sl - first) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
+      if (last && in.p[0] == h->uT) out = B(h, h->tmp1);  // one slice: not in place
+      CHK(slice_inv_right(h, sl, in.get(), out.get()));
+      in = out;
+    }
+    if (in.p[0] != h->uT) CHK(copyd(h, h->uT, in.p[0], (size_t)h->flv * h->nn));
+  }
+  return DQMC_OK;
+}
+
+// compute_forward_udt_block!(mc, s, slice), unequal_time_stack.jl:473-494:
+// Ul diag(Dl) Tl = B_slice .. B_1
+int ut_forward_block(Chain* h, int slice) {
+  const int idx = std::max(0, find_range(h, slice) - 1);
+  CHK(ut_build_forward(h, idx + 1));
+  const Slots F = ut_forward(h);
+  const int target = idx > 0 ? h->re[idx - 1] + 1 : 1;
+  Blocks in = SU(h, F, idx + 1);
+  if (target > slice) {
+    HIPCHK(h, launch_scale_cols(h->tmp1, in.p[0], SD(h, F, idx + 1).p[0], h->n, h->flv, h->st));
+  } else {
+    for (int l = target; l <= slice; l++) {
+      const bool last = l == slice;
+      Blocks out = last ? B(h, h->tmp1) : ((l - target) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
+      CHK(slice_left(h, l, in.get(), out.get(), last ? SD(h, F, idx + 1).get() : nullptr));
+      in = out;
+    }
+  }
+  CHK(udt(h, B(h, h->tmp1).get(), B(h, h->Ul).get(), Bd(h, h->Dl).get(), B(h, h->tmp2).get(), 1));
+  return mul_blocks(h, B(h, h->Tl), B(h, h->tmp2), ST(h, F, idx + 1));
+}
+
+// compute_backward_udt_block!(mc, s, slice), unequal_time_stack.jl:508-531:
+// (Ur diag(Dr) Tr)^T = B_M .. B_{slice+1}
+int ut_backward_block(Chain* h, int slice) {
+  const int idx = find_range(h, slice) + 1;
+  CHK(ut_build_backward(h, idx));
+  const Slots Bk = ut_backward(h);
+  const int target = idx <= h->K ? h->rs[idx - 1] - 1 : h->re[h->K - 1];
+  Blocks in = SU(h, Bk, idx);
+  if (target < slice + 1) {
+    HIPCHK(h, launch_scale_cols(h->tmp1, in.p[0], SD(h, Bk, idx).p[0], h->n, h->flv, h->st));
+  } else {
+    for (int l = target; l >= slice + 1; l--) {
+      const bool last = l == slice + 1;
+      Blocks out = last ? B(h, h->tmp1) : ((target - l) % 2 == 0 ? B(h, h->cur) : B(h, h->tmp2));
+      CHK(slice_dagger_left(h, l, in.get(), out.get(), last ? SD(h, Bk, idx).get() : nullptr));
+      in = out;
+    }
+  }
+  CHK(udt(h, B(h, h->tmp1).get(), B(h, h->Ur).get(), Bd(h, h->Dr).get(), B(h, h->tmp2).get(), 1));
+  return mul_blocks(h, B(h, h->Tr), B(h, h->tmp2), ST(h, Bk, idx));
+}
+
+// C_b = alpha diag(dl_b) op(A_b) diag(dmid_b) op(B_b) diag(dr_b) over the flavour blocks
+int gemm_blocks(Chain* h, int ta, int tb, double alpha, const Blocks& C, const Blocks& A, const Blocks& Bm,
+                const double* dl, int lmode, const double* dmid, int mmode, const double* dr, int rmode,
+                double beta = 0.0) {
+  auto g = batch<double>(ta, tb, alpha, beta);
+  for (int b = 0; b < h->flv; b++) {
+    auto it = item<double>(A.p[b], Bm.p[b], C.p[b], h->n);
+    if (dl) it.dl = diag_vec(dl + (size_t)b * h->n, lmode);
+    if (dmid) it.dmid = diag_vec(dmid + (size_t)b * h->n, mmode);
+    if (dr) it.dr = diag_vec(dr + (size_t)b * h->n, rmode);
+    g.it[g.count++] = it;
+  }
+  return run(h, g);
+}
+
+// calculate_greens_full1!(mc, s, k, l), unequal_time_stack.jl:535-617 (k >= l), into uG
+int ut_full1(Chain* h, int k, int l) {
+  CHK(ut_inverse_block(h, l, k));
+  CHK(ut_forward_block(h, l));
+  CHK(ut_backward_block(h, k));
+  Blocks G = B(h, h->uG), Ul = B(h, h->Ul), Ur = B(h, h->Ur), Tl = B(h, h->Tl), Tr = B(h, h->Tr);
+  Blocks U = B(h, h->uU), T = B(h, h->uT), X = B(h, h->tmp1), Y = B(h, h->tmp2);
+  // G = Dl (Tl Tr') Dr
+  CHK(gemm_blocks(h, 0, 1, 1.0, G, Tl, Tr, h->Dl, kDiagPlain, nullptr, 0, h->Dr, kDiagPlain));
+  CHK(udt(h, G.get(), Tr.get(), Bd(h, h->Dr).get(), G.get(), 0));  // G -> Tr Dr (G P)
+  CHK(mul_blocks(h, Tl, Ul, Tr));                                  // Tl = Ul Tr
+  CHK(rdivp(h, Ur.get(), G.get()));                                // Ur = Ur / G
+  // Tr = 1/max(1, D) U' Tl min(1, Dr)
+  CHK(gemm_blocks(h, 1, 0, 1.0, Tr, U, Tl, h->uD, kDiagMaxInv, nullptr, 0, h->Dr, kDiagMin));
+  // Tr += min(1, D) T Ur 1/max(1, Dr)
+  CHK(gemm_blocks(h, 0, 0, 1.0, Tr, T, Ur, h->uD, kDiagMin, nullptr, 0, h->Dr, kDiagMaxInv, 1.0));
+  CHK(udt(h, Tr.get(), Tl.get(), Bd(h, h->Dl).get(), Tr.get(), 0));  // sum -> Tl Dl (Tr P)
+  // X = diag(1/max(1, Dr)) / (Tr P)
+  HIPCHK(h, launch_diag_matrix(h->tmp1, h->Dr, kDiagMaxInv, h->n, h->flv, h->st));
+  CHK(rdivp(h, X.get(), Tr.get()));
+  // Y = X diag(1/Dl) Tl' diag(1/max(1, D)); G = Ur (Y U')
+  CHK(gemm_blocks(h, 0, 1, 1.0, Y, X, Tl, nullptr, 0, h->Dl, kDiagInv, h->uD, kDiagMaxInv));
+  CHK(gemm_blocks(h, 0, 1, 1.0, X, Y, U, nullptr, 0, nullptr, 0, nullptr, 0));
+  return gemm_blocks(h, 0, 0, 1.0, G, Ur, X, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
+// calculate_greens_full2!(mc, s, k, l), unequal_time_stack.jl:620-697 (k < l), into uG
+int ut_full2(Chain* h, int k, int l) {
+  CHK(ut_inverse_block(h, k, l));
+  CHK(ut_forward_block(h, k));
+  CHK(ut_backward_block(h, l));
+  Blocks G = B(h, h->uG), Ul = B(h, h->Ul), Ur = B(h, h->Ur), Tl = B(h, h->Tl), Tr = B(h, h->Tr);
+  Blocks U = B(h, h->uU), T = B(h, h->uT), X = B(h, h->tmp1), Y = B(h, h->tmp2);
+  CHK(gemm_blocks(h, 0, 1, 1.0, G, Tl, Tr, h->Dl, kDiagPlain, nullptr, 0, h->Dr, kDiagPlain));
+  CHK(udt(h, G.get(), Tr.get(), Bd(h, h->Dr).get(), G.get(), 0));  // G -> Tr Dr (G P)
+  CHK(mul_blocks(h, Tl, Ul, Tr));                                  // Tl = Ul Tr
+  // Ul = 1/max(1, D) U' Tl min(1, Dr)
+  CHK(gemm_blocks(h, 1, 0, 1.0, Ul, U, Tl, h->uD, kDiagMaxInv, nullptr, 0, h->Dr, kDiagMin));
+  // X = (T Ur) / G; Tr = min(1, D) X 1/max(1, Dr) + Ul
+  CHK(mul_blocks(h, X, T, Ur));
+  CHK(rdivp(h, X.get(), G.get()));
+  HIPCHK(h, launch_diag_scale(h->Tr, h->tmp1, h->uD, kDiagMin, h->Dr, kDiagMaxInv, h->Ul, h->n, h->flv, h->st));
+  CHK(udt(h, Tr.get(), Ul.get(), Bd(h, h->Dl).get(), Tr.get(), 0));  // sum -> Ul Dl (Tr P)
+  // X = diag(min(1, Dr)) / (Tr P); Ur = X diag(1/Dl) Ul' diag(min(1, D))
+  HIPCHK(h, launch_diag_matrix(h->tmp1, h->Dr, kDiagMin, h->n, h->flv, h->st));
+  CHK(rdivp(h, X.get(), Tr.get()));
+  CHK(gemm_blocks(h, 0, 1, 1.0, Ur, X, Ul, nullptr, 0, h->Dl, kDiagInv, h->uD, kDiagMin));
+  // G = -Tl (Ur T)
+  CHK(mul_blocks(h, Y, Ur, T));
+  return gemm_blocks(h, 0, 0, -1.0, G, Tl, Y, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
+// calculate_greens(mc, k, l), unequal_time_stack.jl:326-338: the effective G(k <- l) in uG,
+// reused while (k, l) and the field are unchanged.  The chain's temporaries are scratch here;
+// the D of the chain's own G (Dr) is kept for global updates.
+int ut_greens(Chain* h, int k, int l) {
+  CHK(ut_validate(h));
+  if (h->ut_k == k && h->ut_l == l) return DQMC_OK;
+  const size_t nd = (size_t)h->flv * h->n;
+  if (h->dg_valid) CHK(copyd(h, h->uDsave, h->Dr, nd));
+  h->ut_k = h->ut_l = -1;
+  CHK(k >= l ? ut_full1(h, k, l) : ut_full2(h, k, l));
+  if (h->dg_valid) CHK(copyd(h, h->Dr, h->uDsave, nd));
+  h->ut_k = k;
+  h->ut_l = l;
+  return DQMC_OK;
+}
+
+// _greens!(mc, target, source), greens.jl:114-125: out = eT/2+ src eT/2- per flavour block
+int physical_form(Chain* h, const double* src, double* out) {
+  auto g1 = batch<double>(0, 1);
+  auto g2 = batch<double>(1, 0);
+  for (int b = 0; b < h->flv; b++) {
+    g1.it[g1.count++] = item<double>(src + (size_t)b * h->nn, h->eTmh, blk(h->Gx, h, b), h->n);
+    g2.it[g2.count++] = item<double>(h->eTph, blk(h->Gx, h, b), out + (size_t)b * h->nn, h->n);
+  }
+  CHK(run(h, g1));
+  return run(h, g2);
+}
+
+// CombinedGreensIterator(mc; start, stop, recalculate), measurements/greens_iterators.jl:
+// 181-407: for l = start..stop the physical (G(0, l), G(l, 0), G(l, l)), written by `emit`
+// (entry i = l - start, matrix j = 0, 1, 2).  Between recalculations (every `recalculate`
+// slices, from the unequal-time stack) the three are carried as UDT products advanced by one
+// slice per step, re-decomposed every safe_mult slices.
+template <typename Emit>
+int combined_greens(Chain* h, int start, int stop, int rec, Emit emit) {
+  CHK(ut_validate(h));
+  const size_t mats = (size_t)h->flv * h->nn, nv = (size_t)h->flv * h->n;
+  const int n = h->n;
+  double* m[10];
+  for (int i = 0; i < 10; i++) m[i] = h->itm + i * mats;
+  double *Ul = m[0], *Tl = m[1], *Ur = m[2], *Tr = m[3], *uU = m[4], *uT = m[5];
+  double *X = m[6], *Y = m[7], *Z = m[8], *W = m[9];
+  double *Dl = h->itv, *Dr = h->itv + nv, *uD = h->itv + 2 * nv;
+  auto gemm3 = [&](double* C, const double* A, const double* Bm, const double* dmid) -> int {
+    auto g = batch<double>(0, 0);
+    for (int b = 0; b < h->flv; b++) {
+      auto it = item<double>(A + b * h->nn, Bm + b * h->nn, C + b * h->nn, n);
+      if (dmid) it.dmid = diag_vec(dmid + (size_t)b * n);
+      g.it[g.count++] = it;
+    }
+    return run(h, g);
+  };
+  auto advance = [&](int l) -> int {  // Ul = B_l Ul, Tr = Tr B_l^-1, uU = B_l uU, uT = uT B_l^-1
+    CHK(slice_left(h, l, B(h, Ul).get(), B(h, X).get(), nullptr));
+    std::swap(Ul, X);
+    CHK(slice_inv_right(h, l, B(h, Tr).get(), B(h, X).get()));
+    std::swap(Tr, X);
+    CHK(slice_left(h, l, B(h, uU).get(), B(h, X).get(), nullptr));
+    std::swap(uU, X);
+    CHK(slice_inv_right(h, l, B(h, uT).get(), B(h, X).get()));
+    std::swap(uT, X);
+    return DQMC_OK;
+  };
+  auto recalc = [&](int l, int i) -> int {
+    CHK(ut_greens(h, l, 0));
+    CHK(copyd(h, X, h->uG, mats));
+    CHK(ut_greens(h, 0, l));
+    CHK(copyd(h, Y, h->uG, mats));
+    CHK(ut_greens(h, l, l));
+    CHK(copyd(h, Z, h->uG, mats));
+    CHK(emit(i, 0, Y));
+    CHK(emit(i, 1, X));
+    CHK(emit(i, 2, Z));
+    CHK(udt(h, B(h, Z).get(), B(h, uU).get(), Bd(h, uD).get(), B(h, uT).get(), 1));
+    CHK(udt(h, B(h, X).get(), B(h, Ul).get(), Bd(h, Dl).get(), B(h, Tl).get(), 1));
+    return udt(h, B(h, Y).get(), B(h, Ur).get(), Bd(h, Dr).get(), B(h, Tr).get(), 1);
+  };
+  int l;
+  if (start == 0 || start == 1) {
+    // G(0, 0): the chain's own G at current slice 1, else calculate_greens_full1!(0, 0)
+    if (h->cs == 1 && h->prec == DQMC_PRECISION_FP64) {
+      CHK(copyd(h, Y, h->G, mats));
+    } else {
+      CHK(ut_greens(h, 0, 0));
+      CHK(copyd(h, Y, h->uG, mats));
+    }
+    CHK(copyd(h, X, Y, mats));
+    for (int b = 0; b < h->flv; b++) HIPCHK(h, launch_scale_add_identity(X + b * h->nn, 1.0, -1.0, n, h->st));
+    CHK(copyd(h, Z, Y, mats));
+    CHK(udt(h, B(h, Z).get(), B(h, Ul).get(), Bd(h, Dl).get(), B(h, Tl).get(), 1));
+    CHK(copyd(h, uU, Ul, mats));
+    CHK(copyd(h, uD, Dl, nv));
+    CHK(copyd(h, uT, Tl, mats));
+    CHK(udt(h, B(h, X).get(), B(h, Ur).get(), Bd(h, Dr).get(), B(h, Tr).get(), 1));  // G(0, 0) - I
+    if (start == 0) {
+      CHK(emit(0, 0, Y));
+      CHK(emit(0, 1, Y));
+      CHK(emit(0, 2, Y));
+    }
+    l = 1;
+  } else {
+    CHK(recalc(start, 0));
+    l = start + 1;
+  }
+  const int shift = start != 1 ? start : 0;
+  for (; l <= stop; l++) {
+    const int i = l - start;
+    if ((l - shift) % rec == 0) {
+      CHK(recalc(l, i));
+      continue;
+    }
+    CHK(advance(l));
+    if (((l - shift) % rec) % h->sm == 0) {
+      // G(l, 0): Ul Dl Tl, then Ul Dl Tl <- UDT(Ul Dl) Tl
+      HIPCHK(h, launch_scale_cols(X, Ul, Dl, n, h->flv, h->st));
+      CHK(gemm3(Y, X, Tl, nullptr));
+      CHK(emit(i, 1, Y));
+      CHK(udt(h, B(h, X).get(), B(h, Ul).get(), Bd(h, Dl).get(), B(h, Z).get(), 1));
+      CHK(gemm3(X, Z, Tl, nullptr));
+      std::swap(Tl, X);
+      // G(0, l): Ur (Dr Tr), then Ur Dr Tr <- Ur UDT(Dr Tr)
+      HIPCHK(h, launch_diag_scale(X, Tr, Dr, kDiagPlain, nullptr, 0, nullptr, n, h->flv, h->st));
+      CHK(gemm3(Y, Ur, X, nullptr));
+      CHK(emit(i, 0, Y));
+      CHK(udt(h, B(h, X).get(), B(h, Z).get(), Bd(h, Dr).get(), B(h, Tr).get(), 1));
+      CHK(gemm3(X, Ur, Z, nullptr));
+      std::swap(Ur, X);
+      // G(l, l): (uU uD) uT, then uU uD uT <- C UDT(uD T' uT) with C uD T' = UDT(uU uD)
+      HIPCHK(h, launch_scale_cols(X, uU, uD, n, h->flv, h->st));
+      CHK(gemm3(Y, X, uT, nullptr));
+      CHK(emit(i, 2, Y));
+      CHK(udt(h, B(h, X).get(), B(h, Z).get(), Bd(h, uD).get(), B(h, W).get(), 1));
+      CHK(gemm3(X, W, uT, nullptr));
+      HIPCHK(h, launch_diag_scale(W, X, uD, kDiagPlain, nullptr, 0, nullptr, n, h->flv, h->st));
+      CHK(udt(h, B(h, W).get(), B(h, Y).get(), Bd(h, uD).get(), B(h, uT).get(), 1));
+      CHK(gemm3(X, Z, Y, nullptr));
+      std::swap(uU, X);
+    } else {
+      CHK(gemm3(X, Ul, Tl, Dl));
+      CHK(emit(i, 1, X));
+      CHK(gemm3(X, Ur, Tr, Dr));
+      CHK(emit(i, 0, X));
+      CHK(gemm3(X, uU, uT, uD));
+      CHK(emit(i, 2, X));
+    }
+  }
+  return DQMC_OK;
 }
 
 // The counter stream of the window kernels, u_k = SplitMix64(seed, k) >> 11 * 2^-53.
@@ -1115,23 +1623,24 @@ int executor_step(Executor* x, Chain* const* cs, int nc) {
     h->rec = false;
     if (r != DQMC_OK) { x->err = h->err; return r; }
   }
-  // window launches: position k of every chain, grouped by (flavours, window width)
+  // window launches: position k of every chain, grouped by window width (one and two
+  // flavours share a launch)
   size_t kmax = 0;
   for (int i = 0; i < nc; i++) kmax = std::max(kmax, cs[i]->rwin.size());
   std::vector<WindowArgs<double>> wtab;
   for (size_t k = 0; k < kmax; k++)
-    for (int flv = 1; flv <= 2; flv++)
-      for (int W : {16, 32, 64}) {
-        wtab.clear();
-        int mn = 0;
-        for (int i = 0; i < nc; i++)
-          if (cs[i]->rwin.size() > k && cs[i]->flv == flv && cs[i]->win == W) {
-            wtab.push_back(cs[i]->rwin[k]);
-            mn = std::max(mn, cs[i]->n);
-          }
-        if (!wtab.empty())
-          x->launches.push_back({0, x->put(wtab.data(), wtab.size()), (int)wtab.size(), flv, W, mn, 0, 0, 0, 0});
-      }
+    for (int W : {16, 32, 64}) {
+      wtab.clear();
+      int mn = 0, mf = 0;
+      for (int i = 0; i < nc; i++)
+        if (cs[i]->rwin.size() > k && cs[i]->win == W) {
+          wtab.push_back(cs[i]->rwin[k]);
+          mn = std::max(mn, cs[i]->n);
+          mf = std::max(mf, cs[i]->flv);
+        }
+      if (!wtab.empty())
+        x->launches.push_back({0, x->put(wtab.data(), wtab.size()), (int)wtab.size(), mf, W, mn, 0, 0, 0, 0});
+    }
   // propagate: chains grouped by the kinds of their recorded sequences
   std::map<std::vector<int64_t>, std::vector<int>> groups;
   for (int i = 0; i < nc; i++) {
@@ -1438,6 +1947,13 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
     delete h;
     return DQMC_E_HIP;
   }
+  // pinned mirrors for harvest (without them harvest copies synchronously)
+  void* hp = nullptr;
+  if (hipHostMalloc(&hp, sizeof(ChainCounters) + sizeof(double) * (size_t)h->errcap, hipHostMallocDefault) ==
+      hipSuccess) {
+    h->hctr = static_cast<ChainCounters*>(hp);
+    h->herr = reinterpret_cast<double*>(static_cast<char*>(hp) + sizeof(ChainCounters));
+  }
   *out = h;
   return DQMC_OK;
 }
@@ -1448,6 +1964,9 @@ int dqmc_destroy(dqmc_chain_t h) {
   if (h->st) (void)hipStreamSynchronize(h->st);
   collect_times(h);
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
+  if (h->sgx) (void)hipGraphExecDestroy(h->sgx);
+  if (h->sg) (void)hipGraphDestroy(h->sg);
+  if (h->hctr) (void)hipHostFree(h->hctr);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->unif) (void)hipFree(h->unif);
   if (h->dlog) (void)hipFree(h->dlog);
@@ -1527,6 +2046,7 @@ int dqmc_set_field(dqmc_chain_t h, const int8_t* conf, int ld) {
       const int8_t x = conf[(size_t)i + (size_t)l * ld];
       if (x != 1 && x != -1) { h->err = "dqmc_set_field: entries must be +-1"; return DQMC_E_INVALID; }
     }
+  h->fver++;
   HIPCHK(h, hipMemcpy2DAsync(h->conf, h->n, conf, ld, h->n, h->M, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
   return DQMC_OK;
@@ -1565,6 +2085,7 @@ int dqmc_set_field_packed(dqmc_chain_t h, const uint64_t* chunks, int64_t nchunk
     h->err = "dqmc_set_field_packed: bits set beyond the field in the last chunk";
     return DQMC_E_INVALID;
   }
+  h->fver++;
   HIPCHK(h, hipMemcpyAsync(h->cpack, chunks, sizeof(uint64_t) * need, hipMemcpyHostToDevice, h->st));
   HIPCHK(h, launch_unpack_field(h->cpack, count, h->conf, h->st));
   HIPCHK(h, hipStreamSynchronize(h->st));
@@ -1674,22 +2195,113 @@ int dqmc_sweep_spatial(dqmc_chain_t h, int64_t* accepted) {
   return DQMC_OK;
 }
 
+static int sweep_eager(Chain* h) {
+  for (int k = 0; k < 2 * h->M; k++) {
+    CHK(sweep_spatial(h));
+    CHK(propagate(h));
+  }
+  return DQMC_OK;
+}
+
+static void drop_sweep_graph(Chain* h) {
+  if (h->sgx) (void)hipGraphExecDestroy(h->sgx);
+  if (h->sg) (void)hipGraphDestroy(h->sg);
+  h->sgx = nullptr;
+  h->sg = nullptr;
+  h->sg_key.clear();
+}
+
+// Everything a sweep's launches take from the host: the propagate state, which copy of G is
+// which, the error slots' position, the randomness and log buffers, the wait bound.
+static std::vector<uint64_t> sweep_key(const Chain* h) {
+  auto u = [](const void* p) { return (uint64_t)(uintptr_t)p; };
+  return {(uint64_t)h->cs, (uint64_t)(int64_t)h->dir, (uint64_t)h->cr, u(h->G), u(h->Gpp), (uint64_t)h->nerr,
+          (uint64_t)h->seed, u(h->unif), (uint64_t)h->nunif, u(h->dlog), u(h->plog), (uint64_t)h->logcap,
+          (uint64_t)h->wait_limit, (uint64_t)h->win};
+}
+
+// One sweep: replayed from the captured graph when its key matches, captured (then launched)
+// when it does not, and launched kernel by kernel when graphs are off or capture fails.
+static int sweep_once(Chain* h) {
+  const bool graph = h->use_graph && h->prec == DQMC_PRECISION_FP64 && !h->prof && !h->rec &&
+                     h->nerr + 4 * (long long)h->K + 8 < h->errcap;
+  if (!graph) return sweep_eager(h);
+  std::vector<uint64_t> key = sweep_key(h);
+  if (h->sgx && key == h->sg_key) {
+    HIPCHK(h, hipGraphLaunch(h->sgx, h->st));
+    h->fver++;
+    h->acc_valid = false;
+    h->cs = h->sg_cs; h->dir = h->sg_dir; h->cr = h->sg_cr;
+    h->G = h->sg_G; h->Gpp = h->sg_Gpp;
+    h->nerr = h->sg_nerr;
+    return DQMC_OK;
+  }
+  drop_sweep_graph(h);
+  const int cs = h->cs, dir = h->dir, cr = h->cr;
+  double* const G = h->G;
+  double* const Gpp = h->Gpp;
+  const long long nerr = h->nerr;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  int r = DQMC_OK;
+  hipError_t e = hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    r = sweep_eager(h);
+    e = hipStreamEndCapture(h->st, &g);
+    if (r == DQMC_OK && e == hipSuccess && g) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  }
+  if (r != DQMC_OK || e != hipSuccess || !g || !x) {
+    // capture is not available: this chain launches kernel by kernel from now on
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    h->use_graph = false;
+    h->cs = cs; h->dir = dir; h->cr = cr; h->G = G; h->Gpp = Gpp; h->nerr = nerr;
+    return sweep_eager(h);
+  }
+  h->sg = g;
+  h->sgx = x;
+  h->sg_key = key;
+  h->sg_cs = h->cs; h->sg_dir = h->dir; h->sg_cr = h->cr;
+  h->sg_G = h->G; h->sg_Gpp = h->Gpp;
+  h->sg_nerr = h->nerr;
+  HIPCHK(h, hipGraphLaunch(x, h->st));
+  return DQMC_OK;
+}
+
+int dqmc_set_sweep_graph(dqmc_chain_t h, int enable) {
+  if (!h) return DQMC_E_INVALID;
+  h->use_graph = enable != 0;
+  if (!h->use_graph) drop_sweep_graph(h);
+  return DQMC_OK;
+}
+
 int dqmc_sweep(dqmc_chain_t h, int nsweeps, int64_t* accepted) {
   ENTER(h);
   NEED_HOPPING(h);
   if (nsweeps < 0) { h->err = "nsweeps < 0"; return DQMC_E_INVALID; }
   if (h->cs < 1 || h->cs > h->M) { h->err = "dqmc_sweep: build the stack first"; return DQMC_E_STATE; }
   long long a0 = 0, a1 = 0;
-  if (accepted) { HIPCHK(h, hipStreamSynchronize(h->st)); CHK(read_accepted(h, &a0)); }
-  for (int s = 0; s < nsweeps; s++) {
-    for (int k = 0; k < 2 * h->M; k++) {
-      CHK(sweep_spatial(h));
-      CHK(propagate(h));
+  if (accepted) {
+    if (h->acc_valid) {
+      a0 = h->acc_last;
+    } else {
+      HIPCHK(h, hipStreamSynchronize(h->st));
+      CHK(read_accepted(h, &a0));
     }
+  }
+  for (int s = 0; s < nsweeps; s++) {
+    CHK(sweep_once(h));
     h->sweeps++;
+    // a captured sweep writes fixed error slots: collect them before the next one
+    if (h->sgx && s + 1 < nsweeps) CHK(harvest(h));
   }
   CHK(harvest(h));
-  if (accepted) { CHK(read_accepted(h, &a1)); *accepted = a1 - a0; }
+  if (accepted) {
+    if (h->acc_valid) a1 = h->acc_last;
+    else CHK(read_accepted(h, &a1));
+    *accepted = a1 - a0;
+  }
   return DQMC_OK;
 }
 
@@ -1798,6 +2410,113 @@ int dqmc_calculate_greens(dqmc_chain_t h, int slice, double* Gout, int ld) {
   return DQMC_OK;
 }
 
+// greens(mc, k, l) / calculate_greens(mc, k, l), unequal_time_stack.jl:297-338: G(k <- l) from
+// the unequal-time stack of the current field (physical: eT/2+ G eT/2-, greens.jl:114-125).
+int dqmc_unequal_time_greens(dqmc_chain_t h, int k, int l, double* G, int ld, int physical) {
+  ENTER(h);
+  NEED_HOPPING(h);
+  if (!G || ld < h->n || k < 0 || k > h->M || l < 0 || l > h->M) {
+    h->err = "dqmc_unequal_time_greens: null G, ld < n or k, l outside 0..slices";
+    return DQMC_E_INVALID;
+  }
+  CHK(ut_greens(h, k, l));
+  const double* src = h->uG;
+  if (physical) {
+    CHK(physical_form(h, h->uG, h->tmp1));
+    src = h->tmp1;
+  }
+  CHK(download_matrix(h, G, ld, src, h->flv * h->n));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return check_device_errors(h);
+}
+
+// CombinedGreensIterator / TimeIntegral iterated over l = start..stop
+// (measurements/greens_iterators.jl:83-407): out holds 3 (stop - start + 1) matrices of
+// n x (flavors n), G(0, l), G(l, 0), G(l, l) for each l in turn (physical forms).
+int dqmc_combined_greens(dqmc_chain_t h, int start, int stop, int recalculate, double* out, int ld,
+                         int out_on_device) {
+  ENTER(h);
+  NEED_HOPPING(h);
+  if (!out || start < 0 || stop > h->M || start > stop || recalculate < 1 ||
+      (out_on_device ? ld != h->n : ld < h->n)) {
+    h->err = "dqmc_combined_greens: need 0 <= start <= stop <= slices, recalculate >= 1, out != NULL, "
+             "ld >= n (ld == n for device output)";
+    return DQMC_E_INVALID;
+  }
+  const size_t per = (size_t)ld * h->flv * h->n;
+  auto emit = [&](int i, int j, const double* src) -> int {
+    double* dst = out + ((size_t)i * 3 + j) * per;
+    if (out_on_device) return physical_form(h, src, dst);
+    CHK(physical_form(h, src, h->O));
+    return download_matrix(h, dst, ld, h->O, h->flv * h->n);
+  };
+  CHK(combined_greens(h, start, stop, recalculate, emit));
+  HIPCHK(h, hipStreamSynchronize(h->st));
+  return check_device_errors(h);
+}
+
+// charge_density_susceptibility / spin_density_susceptibility(mc, model, dir) measured once
+// for the current field (constructors/charge_density.jl:47, spin_density.jl:54;
+// apply!(::TimeIntegral), measurements/generic.jl:337-370; EachSitePairByDistance, :434-460;
+// finish!, :574-586): the TimeIntegral iteration l = 0..M feeds one accumulation launch per
+// slice, and G never leaves the device.
+int dqmc_density_susceptibility(dqmc_chain_t h, int observable, int recalculate, const int32_t* dir2srctrg,
+                                int n_bravais, int n_basis, int total_flavors, int reduced, double* out) {
+  ENTER(h);
+  NEED_HOPPING(h);
+  if (!dir2srctrg || !out || n_bravais < 1 || n_basis < 1 || (int64_t)n_bravais * n_basis != h->n ||
+      recalculate < 1 || total_flavors < 1 || observable < DQMC_CHARGE_DENSITY || observable > DQMC_SPIN_DENSITY_Z) {
+    h->err = "dqmc_density_susceptibility: need a known observable, n_bravais * n_basis == sites, "
+             "recalculate >= 1, total_flavors >= 1 and non-null tables";
+    return DQMC_E_INVALID;
+  }
+  const int N = n_bravais, Bn = n_basis;
+  for (int64_t k = 0; k < (int64_t)N * N; k++)
+    if (dir2srctrg[k] < 0 || dir2srctrg[k] >= N) {
+      h->err = "dqmc_density_susceptibility: dir2srctrg entries must lie in 0..n_bravais-1";
+      return DQMC_E_INVALID;
+    }
+  CHK(ut_validate(h));
+  const size_t mats = (size_t)h->flv * h->nn, nout = (size_t)N * Bn * Bn;
+  double* G00 = h->itm + 10 * mats;
+  double* ring[3] = {h->itm + 11 * mats, h->itm + 12 * mats, h->itm + 13 * mats};
+  void* dmem = nullptr;
+  HIPCHK(h, hipMalloc(&dmem, sizeof(double) * nout + sizeof(int32_t) * (size_t)N * N));
+  double* acc = static_cast<double*>(dmem);
+  int* d2st = reinterpret_cast<int*>(acc + nout);
+  int r = DQMC_OK;
+  hipError_t e = hipMemcpyAsync(d2st, dir2srctrg, sizeof(int32_t) * (size_t)N * N, hipMemcpyHostToDevice, h->st);
+  if (e == hipSuccess) e = launch_fill(acc, 0.0, (int64_t)nout, h->st);
+  if (e != hipSuccess) { h->err = hipGetErrorString(e); r = DQMC_E_HIP; }
+  if (r == DQMC_OK) {
+    const double* src = nullptr;
+    r = device_greens(h, 1, &src);
+    if (r == DQMC_OK) r = copyd(h, G00, src, mats);
+  }
+  const int M = h->M;
+  auto emit = [&](int i, int j, const double* g) -> int {
+    CHK(physical_form(h, g, ring[j]));
+    if (j != 2) return DQMC_OK;
+    const double w = (i == 0 || i == M ? 0.5 : 1.0) * h->dtau;
+    HIPCHK(h, launch_density_slice(G00, ring[0], ring[1], ring[2], d2st, N, Bn, h->n, h->flv,
+                                   (double)total_flavors, observable, i == 0, reduced, w, acc, h->st));
+    return DQMC_OK;
+  };
+  if (r == DQMC_OK) r = combined_greens(h, 0, M, recalculate, emit);
+  std::vector<double> host(nout);
+  if (r == DQMC_OK) {
+    e = hipMemcpyAsync(host.data(), acc, sizeof(double) * nout, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    if (e != hipSuccess) { h->err = hipGetErrorString(e); r = DQMC_E_HIP; }
+  } else {
+    (void)hipStreamSynchronize(h->st);
+  }
+  (void)hipFree(dmem);
+  if (r != DQMC_OK) return r;
+  for (size_t k = 0; k < nout; k++) out[k] = host[k] / (double)h->n;
+  return check_device_errors(h);
+}
+
 // global_update(mc, model, field) with `proposal` as the new conf (global_updates.jl:191-207):
 //   propose_global_from_conf (:136-169): the inverse D of the last calculate_greens of G
 //   (here 1 / Dr) and inv_det at slice current_slice - 1 = 0 (:64-121), the chain from scratch
@@ -1830,6 +2549,7 @@ int dqmc_global_update(dqmc_chain_t h, const int8_t* proposal, int ld, int* acce
   HIPCHK(h, hipStreamSynchronize(h->st));
   HIPCHK(h, hipMemcpy(dold.data(), h->Dr, sizeof(double) * nd, hipMemcpyDeviceToHost));
   HIPCHK(h, hipMemcpy(cold.data(), h->conf, nc, hipMemcpyDeviceToHost));
+  h->fver++;
   HIPCHK(h, hipMemcpy(h->conf, cnew.data(), nc, hipMemcpyHostToDevice));
   CHK(greens_from_scratch(h, 0, h->Gchk));
   HIPCHK(h, hipStreamSynchronize(h->st));

@@ -37,16 +37,31 @@ __device__ __forceinline__ fx4 mfma16(float a, float b, fx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-constexpr int kTM = 32, kTN = 32, kKC = 128, kLD = 33;
-constexpr int kLoadsPerThread = (kTM + kTN) * kKC / 256;   // 32
+// Tile geometry: TILE x TILE outputs per workgroup, K staged in chunks of KC.  TILE = 32
+// (each wave one 16x16 accumulator, KC = 128) for launches with few tiles; TILE = 64 (each
+// wave a 32x32 quadrant as 2 x 2 accumulators, KC = 64) for launches with many, where it
+// halves the operand traffic per flop.  Every output element accumulates its k-steps of 4
+// in the same order with either geometry, so the two give identical results.
+template <int TILE> struct TileCfg {
+  static constexpr int KC = TILE == 32 ? 128 : 64;
+  static constexpr int LD = TILE + 1;
+  static constexpr int F = TILE / 32;                     // 16x16 fragments per wave and side
+  static constexpr int LPT = 2 * TILE * KC / 256;         // loads per thread per chunk (32)
+};
 
-template <typename T>
-constexpr size_t gemm_lds_bytes() { return sizeof(T) * 2 * 2 * kKC * kLD; }
+template <typename T, int TILE>
+constexpr size_t gemm_lds_bytes() { return sizeof(T) * 2 * 2 * TileCfg<TILE>::KC * TileCfg<TILE>::LD; }
 
-template <typename T, int TA, int TB>
+// TAB: the items come from the device table batch.items (many-chain launches), else from
+// the kernel arguments (a separate instantiation, so the one-chain path reads its items
+// straight from the argument segment)
+template <typename T, int TA, int TB, bool TAB, int TILE>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   typedef typename AccT<T>::type acc_t;
-  const GemmItem<T>& it = batch.items ? batch.items[blockIdx.z] : batch.it[blockIdx.z];
+  constexpr int kTM = TILE, kTN = TILE, kKC = TileCfg<TILE>::KC, kLD = TileCfg<TILE>::LD;
+  constexpr int kF = TileCfg<TILE>::F, kLoadsPerThread = TileCfg<TILE>::LPT;
+  constexpr int kHalf = kLoadsPerThread / 2;
+  const GemmItem<T>& it = TAB ? batch.items[blockIdx.z] : batch.it[blockIdx.z];
   const int M = it.M, N = it.N, K = it.K;
   const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup, before any barrier
@@ -56,7 +71,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   T* Bs = As + 2 * kKC * kLD;                      // [2][kKC][kLD]  Bs[k][n]
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = (wave & 1) * 16, wn = (wave >> 1) * 16;
+  const int wm = (wave & 1) * (16 * kF), wn = (wave >> 1) * (16 * kF);
   const T* __restrict__ A = it.A;
   const T* __restrict__ B = it.B;
   const int lda = it.lda, ldb = it.ldb;
@@ -65,11 +80,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
 
   // element q of this thread in a chunk: q < 16 -> A, else B; index e = tid + 256 (q % 16)
   auto coord = [&](int q, int& mn, int& kk) {
-    const int e = tid + 256 * (q & 15);
-    const bool isA = q < 16;
+    const int e = tid + 256 * (q % kHalf);
+    const bool isA = q < kHalf;
     const bool kfast = isA ? TA == 1 : TB == 0;    // consecutive threads along k in memory
     if (kfast) { kk = e % kKC; mn = e / kKC; }
-    else { mn = e & 31; kk = e >> 5; }
+    else { mn = e % kTM; kk = e / kTM; }
   };
   // two register sets (chunks of even / odd index), so that two chunks can be in flight
   T ra[kLoadsPerThread], rb[kLoadsPerThread];
@@ -84,7 +99,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
       coord(q, mn, kk);
       const int gk = k0 + kk;
       const int gkc = gk < K ? gk : K - 1;
-      if (q < 16) {
+      if (q < kHalf) {
         const int gm = m0 + mn, gmc = gm < M ? gm : M - 1;
         reg[q] = TA == 0 ? A[(size_t)gmc + (size_t)gkc * lda] : A[(size_t)gkc + (size_t)gmc * lda];
       } else {
@@ -93,40 +108,43 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
       }
     }
     if (has_mid) {
-      double f[16];
+      double f[kHalf];
       if (it.dmid.vec) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
+        for (int q = 0; q < kHalf; q++) {
           int mn, kk;
           coord(q, mn, kk);
           const int gk = k0 + kk;
           f[q] = it.dmid.vec[gk < K ? gk : K - 1];
         }
-        if (it.dmid.invert) {
+        if (it.dmid.mode == kDiagInv) {
 #pragma unroll
-          for (int q = 0; q < 16; q++) f[q] = 1.0 / f[q];
+          for (int q = 0; q < kHalf; q++) f[q] = 1.0 / f[q];
+        } else if (it.dmid.mode != kDiagPlain) {
+#pragma unroll
+          for (int q = 0; q < kHalf; q++) f[q] = diag_map(f[q], it.dmid.mode);
         }
       } else {
-        int8_t cf[16];
+        int8_t cf[kHalf];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
+        for (int q = 0; q < kHalf; q++) {
           int mn, kk;
           coord(q, mn, kk);
           const int gk = k0 + kk;
           cf[q] = it.dmid.conf[gk < K ? gk : K - 1];
         }
 #pragma unroll
-        for (int q = 0; q < 16; q++) f[q] = cf[q] > 0 ? it.dmid.pos : it.dmid.neg;
+        for (int q = 0; q < kHalf; q++) f[q] = cf[q] > 0 ? it.dmid.pos : it.dmid.neg;
       }
 #pragma unroll
-      for (int q = 0; q < 16; q++) reg[q] = T(reg[q] * f[q]);
+      for (int q = 0; q < kHalf; q++) reg[q] = T(reg[q] * f[q]);
     }
 #pragma unroll
     for (int q = 0; q < kLoadsPerThread; q++) {
       int mn, kk;
       coord(q, mn, kk);
       const int gk = k0 + kk;
-      const bool ok = q < 16 ? (m0 + mn < M && gk < K) : (n0 + mn < N && gk < K);
+      const bool ok = q < kHalf ? (m0 + mn < M && gk < K) : (n0 + mn < N && gk < K);
       reg[q] = ok ? reg[q] : T(0);
     }
   };
@@ -137,11 +155,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
     for (int q = 0; q < kLoadsPerThread; q++) {
       int mn, kk;
       coord(q, mn, kk);
-      if (q < 16) Ab[kk * kLD + mn] = reg[q];
+      if (q < kHalf) Ab[kk * kLD + mn] = reg[q];
       else Bb[kk * kLD + mn] = reg[q];
     }
   };
-  acc_t acc = acc_t{0, 0, 0, 0};
+  acc_t acc[kF][kF];
+#pragma unroll
+  for (int i = 0; i < kF; i++)
+#pragma unroll
+    for (int j = 0; j < kF; j++) acc[i][j] = acc_t{0, 0, 0, 0};
   const int li = lane & 15, lk = lane >> 4;
   auto compute = [&](int c, int buf) {
     const T* Ab = As + buf * kKC * kLD;
@@ -150,9 +172,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
 #pragma unroll 8
     for (int kk = 0; kk < kKC; kk += 4) {
       if (kk < kend) {
-        const T av = Ab[(kk + lk) * kLD + wm + li];
-        const T bv = Bb[(kk + lk) * kLD + wn + li];
-        acc = mfma16(av, bv, acc);
+        T av[kF], bv[kF];
+#pragma unroll
+        for (int i = 0; i < kF; i++) av[i] = Ab[(kk + lk) * kLD + wm + 16 * i + li];
+#pragma unroll
+        for (int j = 0; j < kF; j++) bv[j] = Bb[(kk + lk) * kLD + wn + 16 * j + li];
+#pragma unroll
+        for (int i = 0; i < kF; i++)
+#pragma unroll
+          for (int j = 0; j < kF; j++) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
       }
     }
   };
@@ -179,20 +207,51 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   const bool has_l = diag_active(it.dl), has_r = diag_active(it.dr), has_add = diag_active(it.dadd);
   T* __restrict__ Cm = it.C;
   const int ldc = it.ldc;
-  const int col = n0 + wn + (lane & 15);
-  if (col < N) {
+#pragma unroll
+  for (int j = 0; j < kF; j++) {
+    const int col = n0 + wn + 16 * j + (lane & 15);
+    if (col >= N) continue;
     const T sr = has_r ? T(diag_at(it.dr, col)) : T(1);
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int row = m0 + wm + mfma_row<T>(lane, r);
-      if (row >= M) continue;
-      T v = alpha * acc[r];
-      if (has_l) v *= T(diag_at(it.dl, row));
-      if (has_r) v *= sr;
-      T* dst = Cm + (size_t)row + (size_t)col * ldc;
-      if (beta != T(0)) v += beta * (*dst);
-      if (has_add && row == col) v += T(diag_at(it.dadd, row));
-      *dst = v;
+    for (int i = 0; i < kF; i++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = m0 + wm + 16 * i + mfma_row<T>(lane, r);
+        if (row >= M) continue;
+        T v = alpha * acc[i][j][r];
+        if (has_l) v *= T(diag_at(it.dl, row));
+        if (has_r) v *= sr;
+        T* dst = Cm + (size_t)row + (size_t)col * ldc;
+        if (beta != T(0)) v += beta * (*dst);
+        if (has_add && row == col) v += T(diag_at(it.dadd, row));
+        *dst = v;
+      }
+    }
+  }
+}
+
+// 64-wide tiles once a launch has at least 8 workgroups of 32 x 32 per CU to spread
+constexpr int64_t kWideTileMin = 8 * 256;
+
+template <typename T, int TILE>
+static void launch_gemm_tile(const GemmBatch<T>& b, int Mx, int Nx, hipStream_t s) {
+  dim3 grid((Mx + TILE - 1) / TILE, (Nx + TILE - 1) / TILE, b.count);
+  dim3 block(256);
+  const size_t lds = gemm_lds_bytes<T, TILE>();
+  const int sel = b.transA * 2 + b.transB;
+  if (b.items) {
+    switch (sel) {
+      case 0: hipLaunchKernelGGL((gemm_kernel<T, 0, 0, true, TILE>), grid, block, lds, s, b); break;
+      case 1: hipLaunchKernelGGL((gemm_kernel<T, 0, 1, true, TILE>), grid, block, lds, s, b); break;
+      case 2: hipLaunchKernelGGL((gemm_kernel<T, 1, 0, true, TILE>), grid, block, lds, s, b); break;
+      default: hipLaunchKernelGGL((gemm_kernel<T, 1, 1, true, TILE>), grid, block, lds, s, b); break;
+    }
+  } else {
+    switch (sel) {
+      case 0: hipLaunchKernelGGL((gemm_kernel<T, 0, 0, false, TILE>), grid, block, lds, s, b); break;
+      case 1: hipLaunchKernelGGL((gemm_kernel<T, 0, 1, false, TILE>), grid, block, lds, s, b); break;
+      case 2: hipLaunchKernelGGL((gemm_kernel<T, 1, 0, false, TILE>), grid, block, lds, s, b); break;
+      default: hipLaunchKernelGGL((gemm_kernel<T, 1, 1, false, TILE>), grid, block, lds, s, b); break;
     }
   }
 }
@@ -214,33 +273,35 @@ static hipError_t launch_gemm_t(const GemmBatch<T>& b, hipStream_t s) {
       if (b.it[i].N > Nx) Nx = b.it[i].N;
     }
   }
-  dim3 grid((Mx + kTM - 1) / kTM, (Nx + kTN - 1) / kTN, b.count);
-  dim3 block(256);
-  const size_t lds = gemm_lds_bytes<T>();
-  const int sel = b.transA * 2 + b.transB;
-  switch (sel) {
-    case 0: hipLaunchKernelGGL((gemm_kernel<T, 0, 0>), grid, block, lds, s, b); break;
-    case 1: hipLaunchKernelGGL((gemm_kernel<T, 0, 1>), grid, block, lds, s, b); break;
-    case 2: hipLaunchKernelGGL((gemm_kernel<T, 1, 0>), grid, block, lds, s, b); break;
-    default: hipLaunchKernelGGL((gemm_kernel<T, 1, 1>), grid, block, lds, s, b); break;
-  }
+  const int64_t tiles32 = (int64_t)((Mx + 31) / 32) * ((Nx + 31) / 32) * b.count;
+  if (Mx >= 64 && Nx >= 64 && tiles32 >= kWideTileMin) launch_gemm_tile<T, 64>(b, Mx, Nx, s);
+  else launch_gemm_tile<T, 32>(b, Mx, Nx, s);
   return hipGetLastError();
 }
 
-hipError_t gemm_kernels_init() {
-  const void* fns[] = {reinterpret_cast<const void*>(gemm_kernel<double, 0, 0>),
-                       reinterpret_cast<const void*>(gemm_kernel<double, 0, 1>),
-                       reinterpret_cast<const void*>(gemm_kernel<double, 1, 0>),
-                       reinterpret_cast<const void*>(gemm_kernel<double, 1, 1>),
-                       reinterpret_cast<const void*>(gemm_kernel<float, 0, 0>),
-                       reinterpret_cast<const void*>(gemm_kernel<float, 0, 1>),
-                       reinterpret_cast<const void*>(gemm_kernel<float, 1, 0>),
-                       reinterpret_cast<const void*>(gemm_kernel<float, 1, 1>)};
+template <typename T, bool TAB, int TILE>
+static hipError_t set_gemm_lds() {
+  const void* fns[] = {reinterpret_cast<const void*>(gemm_kernel<T, 0, 0, TAB, TILE>),
+                       reinterpret_cast<const void*>(gemm_kernel<T, 0, 1, TAB, TILE>),
+                       reinterpret_cast<const void*>(gemm_kernel<T, 1, 0, TAB, TILE>),
+                       reinterpret_cast<const void*>(gemm_kernel<T, 1, 1, TAB, TILE>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+hipError_t gemm_kernels_init() {
+  hipError_t e = set_gemm_lds<double, false, 32>();
+  if (e == hipSuccess) e = set_gemm_lds<double, true, 32>();
+  if (e == hipSuccess) e = set_gemm_lds<double, false, 64>();
+  if (e == hipSuccess) e = set_gemm_lds<double, true, 64>();
+  if (e == hipSuccess) e = set_gemm_lds<float, false, 32>();
+  if (e == hipSuccess) e = set_gemm_lds<float, true, 32>();
+  if (e == hipSuccess) e = set_gemm_lds<float, false, 64>();
+  if (e == hipSuccess) e = set_gemm_lds<float, true, 64>();
+  return e;
 }
 
 hipError_t launch_gemm_f64(const GemmBatch<double>& b, hipStream_t s) { return launch_gemm_t(b, s); }

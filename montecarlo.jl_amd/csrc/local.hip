@@ -988,46 +988,42 @@ static void launch_window_w(const WindowArgs<GT>& a, hipStream_t s) {
 
 // Many chains per launch: workgroup (0, c) runs chain c's window (when it has one in this
 // launch), workgroups (1 + t, c) apply chain c's previous window to tile t.  Each chain's
-// work is exactly that of its own launch of window_kernel / window_apply_kernel.
-template <typename GT, int FLV, int W>
+// work is exactly that of its own launch of window_kernel / window_apply_kernel, with the
+// flavour count of that chain (one and two flavours share the launch).
+template <typename GT, int W>
 __global__ __launch_bounds__(256) void window_multi_kernel(const WindowArgs<GT>* __restrict__ items) {
   extern __shared__ __align__(16) unsigned char smem[];
   const WindowArgs<GT>& a = items[blockIdx.y];
   if (blockIdx.x == 0) {
-    if (a.do_window) window_role<GT, FLV, W>(a, smem);
+    if (a.do_window) {
+      if (a.flv == 1) window_role<GT, 1, W>(a, smem);
+      else window_role<GT, 2, W>(a, smem);
+    }
     return;
   }
   if (a.Yp == nullptr) return;
   const int nt = (a.n + 63) / 64;
   const int tile = (int)blockIdx.x - 1;
-  if (tile >= FLV * nt * nt) return;
+  if (tile >= a.flv * nt * nt) return;
   apply_tile<GT, W>(a.Gd, a.G, a.Sp, a.Yp, a.n, a.wp, tile, smem);
 }
 
-template <int FLV, int W>
-static void launch_window_multi_w(const WindowArgs<double>* items, int count, int max_n, hipStream_t s) {
-  const int nt = (max_n + 63) / 64;
-  const size_t lds = apply_lds<double, W>() > window_lds<double, FLV, W>() ? apply_lds<double, W>()
-                                                                          : window_lds<double, FLV, W>();
-  hipLaunchKernelGGL((window_multi_kernel<double, FLV, W>), dim3(1 + FLV * nt * nt, count), dim3(256), lds, s,
-                     items);
+template <int W>
+constexpr size_t window_multi_lds() {
+  constexpr size_t w1 = window_lds<double, 1, W>(), w2 = window_lds<double, 2, W>(), ap = apply_lds<double, W>();
+  return (w1 > w2 ? (w1 > ap ? w1 : ap) : (w2 > ap ? w2 : ap));
 }
 
-hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int flv, int W, int max_n,
+hipError_t launch_window_multi_f64(const WindowArgs<double>* items, int count, int max_flv, int W, int max_n,
                                    hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  if (count > 65535 || max_n < 1 || max_n > 2048 || flv < 1 || flv > 2) return hipErrorInvalidValue;
-  if (flv == 1) {
-    if (W == 16) launch_window_multi_w<1, 16>(items, count, max_n, s);
-    else if (W == 32) launch_window_multi_w<1, 32>(items, count, max_n, s);
-    else if (W == 64) launch_window_multi_w<1, 64>(items, count, max_n, s);
-    else return hipErrorInvalidValue;
-  } else {
-    if (W == 16) launch_window_multi_w<2, 16>(items, count, max_n, s);
-    else if (W == 32) launch_window_multi_w<2, 32>(items, count, max_n, s);
-    else if (W == 64) launch_window_multi_w<2, 64>(items, count, max_n, s);
-    else return hipErrorInvalidValue;
-  }
+  if (count > 65535 || max_n < 1 || max_n > 2048 || max_flv < 1 || max_flv > 2) return hipErrorInvalidValue;
+  const int nt = (max_n + 63) / 64;
+  const dim3 grid(1 + max_flv * nt * nt, count);
+  if (W == 16) hipLaunchKernelGGL((window_multi_kernel<double, 16>), grid, dim3(256), window_multi_lds<16>(), s, items);
+  else if (W == 32) hipLaunchKernelGGL((window_multi_kernel<double, 32>), grid, dim3(256), window_multi_lds<32>(), s, items);
+  else if (W == 64) hipLaunchKernelGGL((window_multi_kernel<double, 64>), grid, dim3(256), window_multi_lds<64>(), s, items);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
@@ -1070,12 +1066,9 @@ hipError_t local_kernels_init() {
                        reinterpret_cast<const void*>(window_apply_kernel<float, 16>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 32>),
                        reinterpret_cast<const void*>(window_apply_kernel<float, 64>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 16>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 32>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 1, 64>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 16>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 32>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 2, 64>)};
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 16>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 32>),
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 64>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;

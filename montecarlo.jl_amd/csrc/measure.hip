@@ -82,4 +82,67 @@ hipError_t launch_measure(const double* G, const double* T, int n, int flv, doub
   return hipGetLastError();
 }
 
+// One time slice of a density susceptibility (apply! of measurements/generic.jl:337-370 with
+// EachSitePairByDistance, :434-460): acc[dir + N (b1 + B b2)] += w sum_src K(i, j),
+// i = src + b1 N, j = dir2srctrg[dir][src] + b2 N, for physical G00 (the chain's current G),
+// G0l, Gl0, Gll (flv blocks).  K by observable (kDensityCharge: full_cdc_kernel /
+// reduced_cdc_kernel, constructors/charge_density.jl:62-170, over the flavour pairs;
+// kDensitySpinX / Y / Z: full_sdc_*_kernel / reduced_sdc_*_kernel,
+// constructors/spin_density.jl:68-330, called once).  One flavour block is the reference's
+// DiagonallyRepeatingMatrix form (tf = total flavours), two its BlockDiagonal form.  One wave
+// per output entry; each lane sums its src terms in order and a fixed butterfly reduces them,
+// so the result is reproducible bit for bit.
+__global__ __launch_bounds__(64) void density_slice_kernel(
+    const double* __restrict__ G00, const double* __restrict__ G0l, const double* __restrict__ Gl0,
+    const double* __restrict__ Gll, const int* __restrict__ d2st, int N, int B, int n, int flv, double tf,
+    int observable, int first_slice, int reduced, double w, double* __restrict__ acc) {
+  const int e = blockIdx.x;               // dir + N (b1 + B b2)
+  const int dir = e % N, b1 = (e / N) % B, b2 = e / (N * B);
+  const size_t nn = (size_t)n * n;
+  double sum = 0.0;
+  for (int s = threadIdx.x; s < N; s += 64) {
+    const int i = s + b1 * N;
+    const int j = d2st[(size_t)dir * N + s] + b2 * N;
+    const double id = (i == j && first_slice) ? 1.0 : 0.0;
+    const size_t ij = (size_t)i + (size_t)j * n, ji = (size_t)j + (size_t)i * n;
+    const size_t ii = (size_t)i * (n + 1), jj = (size_t)j * (n + 1);
+    double v = 0.0;
+    if (flv == 1) {
+      if (observable == kDensityCharge) {
+        v = tf * (id - G0l[ji]) * Gl0[ij];
+        if (!reduced) v += tf * tf * (1.0 - Gll[ii]) * (1.0 - G00[jj]);
+      } else {
+        v = 2.0 * (id - G0l[ji]) * Gl0[ij];
+      }
+    } else if (observable == kDensityCharge) {
+      for (int f1 = 0; f1 < flv; f1++) {
+        if (!reduced)
+          for (int f2 = 0; f2 < flv; f2++) v += (1.0 - Gll[f1 * nn + ii]) * (1.0 - G00[f2 * nn + jj]);
+        v += (id - G0l[f1 * nn + ji]) * Gl0[f1 * nn + ij];
+      }
+    } else if (observable == kDensitySpinZ) {
+      if (!reduced) {
+        const double a1 = 1.0 - Gll[ii], a2 = 1.0 - Gll[nn + ii];
+        const double c1 = 1.0 - G00[jj], c2 = 1.0 - G00[nn + jj];
+        v = a1 * c1 - a1 * c2 - a2 * c1 + a2 * c2;
+      }
+      v += (id - G0l[ji]) * Gl0[ij] + (id - G0l[nn + ji]) * Gl0[nn + ij];
+    } else {  // spin x and y: the same for block-diagonal G
+      v = (id - G0l[ji]) * Gl0[nn + ij] + (id - G0l[nn + ji]) * Gl0[ij];
+    }
+    sum += v;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if (threadIdx.x == 0) acc[e] += w * sum;
+}
+
+hipError_t launch_density_slice(const double* G00, const double* G0l, const double* Gl0, const double* Gll,
+                                const int* d2st, int N, int B, int n, int flv, double total_flavors, int observable,
+                                int first_slice, int reduced, double w, double* acc, hipStream_t s) {
+  hipLaunchKernelGGL(density_slice_kernel, dim3((unsigned)(N * B * B)), dim3(64), 0, s, G00, G0l, Gl0, Gll, d2st,
+                     N, B, n, flv, total_flavors, observable, first_slice, reduced, w, acc);
+  return hipGetLastError();
+}
+
 }  // namespace dqmc

@@ -86,8 +86,12 @@ __host__ __device__ static int udt_panel_width(int rows) {
   return w;
 }
 
+// TAB (all UDT and rdivp kernels): items from the device table batch.items with their own n
+// and panel (many-chain launches), else from the kernel arguments (a separate instantiation
+// for the one-chain path)
+template <bool TAB>
 __global__ __launch_bounds__(256) void udt_norms_kernel(UdtBatch batch) {
-  const bool tab = batch.items != nullptr;
+  constexpr bool tab = TAB;
   const UdtItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
   const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -346,9 +350,9 @@ __device__ __forceinline__ void panel_steps_block(double (&a)[R][WM], double* re
 // with the panel rows in registers (R rows x WM columns per thread), compact-WY factor T.
 // Per column step one block reduction gives ||x||^2 and x^T a_c for the later columns:
 // with v = (x + nu e_j) / (x_j + nu), v^T a_c = a_jc + (x_{>j}^T a_c{>j}) / (x_j + nu).
-template <int R, int WM>
+template <int R, int WM, bool TAB>
 __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, int p0a, int wa, int lasta) {
-  const bool tab = batch.items != nullptr;
+  constexpr bool tab = TAB;
   const UdtItem it = tab ? batch.items[blockIdx.x] : batch.it[blockIdx.x];
   const int n = tab ? it.n : batch.n;
   const int p0 = tab ? it.p0 : p0a, w = tab ? it.w : wa, last = tab ? it.last : lasta;
@@ -782,11 +786,12 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
 // Panel p0's block reflector applied in one launch: workgroups [0, nA) update the trailing
 // 16-column strips of A from column c0 (with residual norms), the rest accumulate Q on
 // 16-row strips of U (U <- U H, forward: U holds H_1 ... H_k after panel k).
+template <bool TAB>
 __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int nA) {
   __shared__ double red[4][4][64];
   __shared__ double nred[4][16];
   const int bx = blockIdx.x;
-  if (batch.items) {
+  if constexpr (TAB) {
     // item table: this item's own panel; workgroups beyond its strips leave at once
     const UdtItem& it = batch.items[blockIdx.y];
     const int n = it.n, c0i = it.p0 + it.w;
@@ -802,8 +807,9 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
 
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
 // is copied), one wave per column.
+template <bool TAB>
 __global__ __launch_bounds__(256) void udt_finish_kernel(UdtBatch batch) {
-  const bool tab = batch.items != nullptr;
+  constexpr bool tab = TAB;
   const UdtItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
   const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -866,15 +872,19 @@ hipError_t udt_check(const UdtBatch& b) {
 }
 
 hipError_t launch_udt_norms(const UdtBatch& b, hipStream_t s) {
-  const int n = b.items ? b.max_n : b.n;
-  hipLaunchKernelGGL(udt_norms_kernel, dim3((n + 3) / 4, b.count), dim3(256), 0, s, b);
+  if (b.items) hipLaunchKernelGGL(udt_norms_kernel<true>, dim3((b.max_n + 3) / 4, b.count), dim3(256), 0, s, b);
+  else hipLaunchKernelGGL(udt_norms_kernel<false>, dim3((b.n + 3) / 4, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
 template <int R, int WM>
 static void launch_panel_t(const UdtBatch& b, int p0, int w, int last, hipStream_t s) {
-  hipLaunchKernelGGL((udt_panel_kernel<R, WM>), dim3(b.count), dim3(kQrThreads), udt_panel_lds_bytes(b.n), s, b,
-                     p0, w, last);
+  if (b.items)
+    hipLaunchKernelGGL((udt_panel_kernel<R, WM, true>), dim3(b.count), dim3(kQrThreads), udt_panel_lds_bytes(b.n),
+                       s, b, p0, w, last);
+  else
+    hipLaunchKernelGGL((udt_panel_kernel<R, WM, false>), dim3(b.count), dim3(kQrThreads), udt_panel_lds_bytes(b.n),
+                       s, b, p0, w, last);
 }
 
 hipError_t launch_udt_panel(const UdtBatch& b, int p0, int w, int last, hipStream_t s) {
@@ -894,13 +904,13 @@ hipError_t launch_udt_reflect(const UdtBatch& b, int p0, int w, hipStream_t s) {
   const int c0 = p0 + w;
   const int nA = c0 < b.n ? (b.n - c0 + 15) / 16 : 0;
   const int nQ = (b.n + 15) / 16;
-  hipLaunchKernelGGL(udt_reflect_kernel, dim3(nA + nQ, b.count), dim3(256), 0, s, b, p0, w, c0, nA);
+  hipLaunchKernelGGL(udt_reflect_kernel<false>, dim3(nA + nQ, b.count), dim3(256), 0, s, b, p0, w, c0, nA);
   return hipGetLastError();
 }
 
 hipError_t launch_udt_finish(const UdtBatch& b, hipStream_t s) {
-  const int n = b.items ? b.max_n : b.n;
-  hipLaunchKernelGGL(udt_finish_kernel, dim3((n + 3) / 4, b.count), dim3(256), 0, s, b);
+  if (b.items) hipLaunchKernelGGL(udt_finish_kernel<true>, dim3((b.max_n + 3) / 4, b.count), dim3(256), 0, s, b);
+  else hipLaunchKernelGGL(udt_finish_kernel<false>, dim3((b.n + 3) / 4, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
@@ -923,7 +933,7 @@ hipError_t launch_udt_panel_tab(const UdtBatch& b, int R, hipStream_t s) {
 
 hipError_t launch_udt_reflect_tab(const UdtBatch& b, int max_grid, hipStream_t s) {
   if (!b.items || b.count < 1 || max_grid < 1) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(udt_reflect_kernel, dim3(max_grid, b.count), dim3(256), 0, s, b, 0, 0, 0, 0);
+  hipLaunchKernelGGL(udt_reflect_kernel<true>, dim3(max_grid, b.count), dim3(256), 0, s, b, 0, 0, 0, 0);
   return hipGetLastError();
 }
 
@@ -948,8 +958,9 @@ hipError_t launch_udt(const UdtBatch& b, hipStream_t s) {
 // coupling to the already solved columns, X_{<J} T_{<J,J}, is a 16 x 16 x j0 product on
 // the matrix cores, then the 16 x 16 triangular block is solved by 16 lanes (one row each)
 // in the reference's order: x_j = (o_j - sum_{k<j} x_k T_kj) / T_jj.
+template <bool TAB>
 __global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
-  const bool tab = batch.items != nullptr;
+  constexpr bool tab = TAB;
   const RdivpItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
   const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1014,13 +1025,14 @@ __global__ __launch_bounds__(256) void rdivp_kernel(RdivpBatch batch) {
 // stays there, so a panel's coupling X_{<J} T_{<J,J} reads only T from global memory.
 static size_t rdivp_strip_lds_bytes(int n) { return ((size_t)n * 17 + 2 * 16 * 17) * 8 + (size_t)n * 4; }
 
+template <bool TAB>
 __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
-  const bool tab = batch.items != nullptr;
+  constexpr bool tab = TAB;
   const RdivpItem it = tab ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
   const int n = tab ? it.n : batch.n;
   const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * 16;
-  if (r0 >= n) return;   // item-table launches size the grid for the largest item
+  if (TAB && r0 >= n) return;   // item-table launches size the grid for the largest item
   extern __shared__ __align__(16) unsigned char smem[];
   double* X = reinterpret_cast<double*>(smem);   // [n][17]
   double* Ysh = X + (size_t)n * 17;              // [16][17]
@@ -1160,27 +1172,32 @@ hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
     // and larger ones apart); the grid and LDS are sized for the largest item
     if (b.max_n < 1 || b.max_n > 2048) return hipErrorInvalidValue;
     if (b.max_n <= kRdivpStripMaxN)
-      hipLaunchKernelGGL(rdivp_strip_kernel, dim3((b.max_n + 15) / 16, b.count), dim3(64),
+      hipLaunchKernelGGL(rdivp_strip_kernel<true>, dim3((b.max_n + 15) / 16, b.count), dim3(64),
                          rdivp_strip_lds_bytes(b.max_n), s, b);
     else
-      hipLaunchKernelGGL(rdivp_kernel, dim3((b.max_n + 63) / 64, b.count), dim3(256), 0, s, b);
+      hipLaunchKernelGGL(rdivp_kernel<true>, dim3((b.max_n + 63) / 64, b.count), dim3(256), 0, s, b);
     return hipGetLastError();
   }
   if (b.count > kMaxUdtItems) return hipErrorInvalidValue;
   if (b.n <= kRdivpStripMaxN)
-    hipLaunchKernelGGL(rdivp_strip_kernel, dim3((b.n + 15) / 16, b.count), dim3(64), rdivp_strip_lds_bytes(b.n), s,
-                       b);
+    hipLaunchKernelGGL(rdivp_strip_kernel<false>, dim3((b.n + 15) / 16, b.count), dim3(64),
+                       rdivp_strip_lds_bytes(b.n), s, b);
   else
-    hipLaunchKernelGGL(rdivp_kernel, dim3((b.n + 63) / 64, b.count), dim3(256), 0, s, b);
+    hipLaunchKernelGGL(rdivp_kernel<false>, dim3((b.n + 63) / 64, b.count), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
 hipError_t qr_kernels_init() {
-  const void* fns[] = {reinterpret_cast<const void*>(udt_panel_kernel<1, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16>),
-                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8>),
-                       reinterpret_cast<const void*>(rdivp_strip_kernel)};
+  const void* fns[] = {reinterpret_cast<const void*>(udt_panel_kernel<1, 16, false>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16, false>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16, false>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8, false>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<1, 16, true>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<2, 16, true>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<4, 16, true>),
+                       reinterpret_cast<const void*>(udt_panel_kernel<8, 8, true>),
+                       reinterpret_cast<const void*>(rdivp_strip_kernel<false>),
+                       reinterpret_cast<const void*>(rdivp_strip_kernel<true>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;

@@ -9,11 +9,14 @@
 // every other device buffer of the chain) fits the memory budget.
 //   - An admitted request is prepared on a thread of its own (create, set hopping / field /
 //     seed, build_stack on the chain's own stream), then joins the executor.
-//   - The executor thread runs lockstep slice visits (sweep_spatial + propagate) of every
-//     FP64 chain that has joined, one launch per kernel kind covering all of them
-//     (ensemble.h); a chain leaves after its 2 M nsweeps visits and a finishing thread reads
+//   - An executor thread per size class (N <= 256, <= 576, <= 1024, larger), each with its
+//     own stream, runs lockstep slice visits (sweep_spatial + propagate) of every FP64 chain
+//     of its class that has joined, one launch per kernel kind covering all of them
+//     (ensemble.h).  A lockstep step lasts as long as its largest chain's, so the classes
+//     keep small requests from waiting on the largest ones; the classes' streams overlap on
+//     the device.  A chain leaves after its 2 M nsweeps visits and a finishing thread reads
 //     back field, G and statistics and frees it.  Requests that arrive while others run
-//     join at the next step, so their work shares the running launches.
+//     join at the next step of their class, so their work shares the running launches.
 //   - Mixed-precision requests run on their own thread and stream (dqmc_sweep).
 // A chain's result does not depend on what else runs: every kernel of a chain reads and
 // writes only that chain's buffers, in the chain's own order.
@@ -74,9 +77,11 @@ struct dqmc_server_s {
   int waiters = 0;        // threads inside dqmc_server_wait (destroy waits for them to leave)
   bool stop = false;
   std::thread scheduler;
-  // the executor: built chains wait in `joining`, then run in lockstep steps
-  std::thread executor;
-  std::deque<std::shared_ptr<Request>> joining;
+  // the executors (one per size class): built chains wait in `joining`, then run in
+  // lockstep steps
+  static constexpr int kClasses = 4;
+  std::thread executor[kClasses];
+  std::deque<std::shared_ptr<Request>> joining[kClasses];
   int helpers = 0;        // preparing / finishing threads still running
 };
 
@@ -154,19 +159,22 @@ void complete(dqmc_server_s* s, const std::shared_ptr<Request>& r, bool ok) {
   s->cv.notify_all();
 }
 
-void executor_loop(dqmc_server_s* s) {
+int size_class(int n) { return n <= 256 ? 0 : n <= 576 ? 1 : n <= 1024 ? 2 : 3; }
+
+void executor_loop(dqmc_server_s* s, int cls) {
   dqmc::Ensemble* ens = nullptr;
   const int rc = dqmc::ensemble_create(s->cfg.device, &ens);
   std::vector<std::shared_ptr<Request>> act;
   std::vector<dqmc_chain_t> hs;
+  auto& joining = s->joining[cls];
   for (;;) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
-      s->cv.wait(lk, [&]() { return !act.empty() || !s->joining.empty() || (s->stop && s->helpers == 0); });
-      if (act.empty() && s->joining.empty() && s->stop && s->helpers == 0) break;
-      while (!s->joining.empty()) {
-        act.push_back(s->joining.front());
-        s->joining.pop_front();
+      s->cv.wait(lk, [&]() { return !act.empty() || !joining.empty() || (s->stop && s->helpers == 0); });
+      if (act.empty() && joining.empty() && s->stop && s->helpers == 0) break;
+      while (!joining.empty()) {
+        act.push_back(joining.front());
+        joining.pop_front();
       }
     }
     // newly joined chains attach (a failed attach fails the request)
@@ -247,7 +255,8 @@ void scheduler_loop(dqmc_server_s* s) {
         return;
       }
       std::lock_guard<std::mutex> g(s->mu);
-      s->joining.push_back(r);   // the executor owns it now; `helpers` drops when it finishes
+      // the executor of its size class owns it now; `helpers` drops when it finishes
+      s->joining[size_class(r->cfg.n_sites)].push_back(r);
       s->cv.notify_all();
     }).detach();
   }
@@ -271,7 +280,7 @@ int dqmc_server_create(const dqmc_server_config_t* cfg, dqmc_server_t* out) {
   auto* s = new dqmc_server_s();
   s->cfg = *cfg;
   s->scheduler = std::thread(scheduler_loop, s);
-  s->executor = std::thread(executor_loop, s);
+  for (int c = 0; c < dqmc_server_s::kClasses; c++) s->executor[c] = std::thread(executor_loop, s, c);
   *out = s;
   return DQMC_OK;
 }
@@ -398,7 +407,7 @@ int dqmc_server_destroy(dqmc_server_t s) {
     s->cv.wait(lk, [&]() { return s->active == 0 && s->waiters == 0 && s->helpers == 0; });
   }
   s->scheduler.join();
-  s->executor.join();
+  for (auto& t : s->executor) t.join();
   delete s;
   return DQMC_OK;
 }

@@ -194,6 +194,46 @@ hipError_t launch_maxabsdiff(const double* a, const float* a32, const double* b,
   return hipGetLastError();
 }
 
+// out_b = diag(map(dl_b)) in_b diag(map(dr_b)) [+ add_b] for nmat consecutive n x n blocks,
+// with dl_b = dl + b n and dr_b = dr + b n (either may be null: no scaling on that side)
+__global__ void diag_scale_kernel(double* __restrict__ out, const double* __restrict__ in,
+                                  const double* __restrict__ dl, int lmode, const double* __restrict__ dr,
+                                  int rmode, const double* __restrict__ add, int n, int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / nn, i = (e - b * nn) % n, j = (e - b * nn) / n;
+    double v = in[e];
+    if (dl) v = diag_map(dl[b * n + i], lmode) * v;
+    if (dr) v = v * diag_map(dr[b * n + j], rmode);
+    if (add) v = v + add[e];
+    out[e] = v;
+  }
+}
+hipError_t launch_diag_scale(double* out, const double* in, const double* dl, int lmode, const double* dr,
+                             int rmode, const double* add, int n, int nmat, hipStream_t s) {
+  const int64_t total = (int64_t)n * n * nmat;
+  hipLaunchKernelGGL(diag_scale_kernel, grid_for(total), dim3(256), 0, s, out, in, dl, lmode, dr, rmode,
+                     add, n, total);
+  return hipGetLastError();
+}
+
+// A_b = diag(map(v_b)) for nmat consecutive n x n blocks (v_b = v + b n)
+__global__ void diag_matrix_kernel(double* __restrict__ A, const double* __restrict__ v, int mode, int n,
+                                   int64_t total) {
+  const int64_t nn = (int64_t)n * n;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / nn, i = (e - b * nn) % n, j = (e - b * nn) / n;
+    A[e] = i == j ? diag_map(v[b * n + i], mode) : 0.0;
+  }
+}
+hipError_t launch_diag_matrix(double* A, const double* v, int mode, int n, int nmat, hipStream_t s) {
+  const int64_t total = (int64_t)n * n * nmat;
+  hipLaunchKernelGGL(diag_matrix_kernel, grid_for(total), dim3(256), 0, s, A, v, mode, n, total);
+  return hipGetLastError();
+}
+
 // Many-chain forms: item k of the table is handled by the workgroups with blockIdx.y = k.
 __global__ __launch_bounds__(256) void elem_multi_kernel(const ElemItem* __restrict__ items) {
   const ElemItem& it = items[blockIdx.y];

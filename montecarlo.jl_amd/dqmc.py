@@ -168,10 +168,26 @@ class DQMC:
         self.recorder.push(self.field, self.last_sweep, chunks)
 
     def _measure(self):
-        if self.device_measurements and hasattr(self.stack, "measure_equal_time"):
-            self.measurements["equal_time"].push_values(self.stack.measure_equal_time(self.model.U))
-        else:
-            self.measurements["equal_time"].push(self.greens(physical=True))
+        for key, m in self.measurements.items():
+            if key == "equal_time":
+                if self.device_measurements and hasattr(self.stack, "measure_equal_time"):
+                    m.push_values(self.stack.measure_equal_time(self.model.U))
+                else:
+                    m.push(self.greens(physical=True))
+            else:
+                m.measure(self)
+
+    def __setitem__(self, key: str, measurement):
+        """mc[:key] = measurement (DQMC_interface.jl): measured every measure_rate sweeps
+        after thermalisation."""
+        if key == "equal_time":
+            raise KeyError("'equal_time' is the built-in equal-time set")
+        if key in self._restored:
+            measurement.restore(self._restored.pop(key))
+        self.measurements[key] = measurement
+
+    def __getitem__(self, key: str):
+        return self.measurements[key]
 
     def run(self, *, verbose: bool = True, sweeps: int | None = None,
             thermalization: int | None = None, min_update_rate: float = 0.001,

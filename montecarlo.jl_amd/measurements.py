@@ -97,3 +97,75 @@ class MeasurementSet:
         self.count = int(state["count"])
         self.sums = {k[4:]: (float(v) if np.ndim(v) == 0 else np.array(v))
                      for k, v in state.items() if k.startswith("sum/")}
+
+
+class GreensAt:
+    """GreensAt(k, l) (measurements/greens_iterators.jl:65-77): the time-displaced G(k <- l)."""
+
+    def __init__(self, k: int, l: int | None = None):
+        self.k, self.l = k, (k if l is None else l)
+
+
+class TimeIntegral:
+    """TimeIntegral(recalculate) (measurements/greens_iterators.jl:83-104); -1 = 2 safe_mult."""
+
+    def __init__(self, recalculate: int = -1):
+        self.recalculate = recalculate
+
+
+class DeviceMeasurement:
+    """A measurement evaluated by the library for the chain's current field, once per
+    measure call, averaged like the reference's observables (LogBinner means)."""
+
+    def __init__(self, name: str, evaluate):
+        self.name = name
+        self._evaluate = evaluate
+        self.count = 0
+        self.sum = None
+
+    def measure(self, mc):
+        v = np.asarray(self._evaluate(mc))
+        self.sum = v.copy() if self.sum is None else self.sum + v
+        self.count += 1
+
+    def mean(self):
+        return self.sum / max(self.count, 1)
+
+    def state(self) -> dict:
+        return {"count": np.asarray(self.count), "sum": np.asarray(0.0 if self.sum is None else self.sum)}
+
+    def restore(self, state: dict):
+        self.count = int(state["count"])
+        self.sum = None if self.count == 0 else np.array(state["sum"])
+
+
+def _recalc(mc, it):
+    return None if it.recalculate == -1 else it.recalculate
+
+
+def greens_measurement(mc, model=None, greens_iterator=None):
+    """greens_measurement(mc, model, GreensAt(k, l)) (constructors/greens.jl): the mean of
+    greens(mc, k, l), (flavours, N, N).  Without an iterator: the equal-time G."""
+    if greens_iterator is None:
+        return DeviceMeasurement("G", lambda m: m.greens(physical=True))
+    k, l = greens_iterator.k, greens_iterator.l
+    return DeviceMeasurement(f"G({k},{l})", lambda m: m.stack.unequal_time_greens(k, l))
+
+
+def charge_density_susceptibility(mc, model=None, *, reduced: bool = False,
+                                  greens_iterator: TimeIntegral | None = None):
+    """charge_density_susceptibility(mc, model) (constructors/charge_density.jl:47) with
+    EachSitePairByDistance: (Ndir, B, B) per measurement; reduced = reduced_cdc_kernel."""
+    it = greens_iterator or TimeIntegral()
+    return DeviceMeasurement("CDS", lambda m: m.stack.density_susceptibility(
+        m.model.l, "charge", recalculate=_recalc(m, it), reduced=reduced))
+
+
+def spin_density_susceptibility(mc, model=None, dir: str = "z", *, reduced: bool = False,
+                                greens_iterator: TimeIntegral | None = None):
+    """spin_density_susceptibility(mc, model, dir) (constructors/spin_density.jl:54)."""
+    if dir not in ("x", "y", "z"):
+        raise ValueError("dir must be :x, :y or :z")
+    it = greens_iterator or TimeIntegral()
+    return DeviceMeasurement(f"SDS{dir}", lambda m: m.stack.density_susceptibility(
+        m.model.l, dir, recalculate=_recalc(m, it), reduced=reduced))
