@@ -488,6 +488,14 @@ def main():
             roofline["avg_launch_us_rocprof"] = rp
             roofline["events_over_rocprof"] = kernels[dom]["avg_us"] / rp
             roofline["frac_rocprof"] = nbytes / (rp * 1e-6) / 1e9 / PEAK["hbm_gbs"]
+        # the slice-product GEMM (two n x n items): measured HBM bytes against reading A and
+        # B and writing C once per item
+        gt = measured_traffic("gemm_kernel")
+        if gt:
+            alg = 3.0 * flv * n * n * 8
+            roofline["gemm_traffic"] = {"kernel": "gemm_kernel<double, 1, 0, false, 32>",
+                                        "traffic_bytes_per_launch": gt,
+                                        "algorithmic_bytes_per_launch": alg, "ratio": gt / alg}
     else:
         peak = PEAK["fp32_mfma_tflops"] if dom == "gemm_f32" else PEAK["fp64_mfma_tflops"]
         achieved = kernels[dom]["tflops"]
