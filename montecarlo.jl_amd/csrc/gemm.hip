@@ -52,6 +52,28 @@ template <int TILE> struct TileCfg {
 template <typename T, int TILE>
 constexpr size_t gemm_lds_bytes() { return sizeof(T) * 2 * 2 * TileCfg<TILE>::KC * TileCfg<TILE>::LD; }
 
+// XCD-aware tile order.  Workgroups go to the 8 XCDs round-robin by linear id, and each XCD
+// has its own L2: with the plain order, XCD k gets the tiles of row block k, so every XCD
+// fetches every column panel of B.  When the grid splits evenly, the tiles are renumbered
+// so that XCD k holds a contiguous run of 4 x 4 super-tiles (of one item), whose 4 row and
+// 4 column panels then come from its own L2.  A permutation of the tiles: results are equal.
+constexpr int kXcds = 8, kSuper = 4;
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+  const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+  bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
+  const int per_item = gx * gy;
+  const int total = per_item * gz;
+  if (gx % kSuper || gy % kSuper || total % kXcds) return;
+  const int hw = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int logical = (hw % kXcds) * (total / kXcds) + hw / kXcds;
+  bz = logical / per_item;
+  const int t = logical % per_item;
+  const int sb = t / (kSuper * kSuper), w = t % (kSuper * kSuper);
+  const int sbx = sb % (gx / kSuper), sby = sb / (gx / kSuper);
+  bx = sbx * kSuper + w % kSuper;
+  by = sby * kSuper + w / kSuper;
+}
+
 // TAB: the items come from the device table batch.items (many-chain launches), else from
 // the kernel arguments (a separate instantiation, so the one-chain path reads its items
 // straight from the argument segment)
@@ -61,9 +83,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
   constexpr int kTM = TILE, kTN = TILE, kKC = TileCfg<TILE>::KC, kLD = TileCfg<TILE>::LD;
   constexpr int kF = TileCfg<TILE>::F, kLoadsPerThread = TileCfg<TILE>::LPT;
   constexpr int kHalf = kLoadsPerThread / 2;
-  const GemmItem<T>& it = TAB ? batch.items[blockIdx.z] : batch.it[blockIdx.z];
+  int bx, by, bz;
+  xcd_tile(bx, by, bz);
+  const GemmItem<T>& it = TAB ? batch.items[bz] : batch.it[bz];
   const int M = it.M, N = it.N, K = it.K;
-  const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+  const int m0 = bx * kTM, n0 = by * kTN;
   if (m0 >= M || n0 >= N) return;  // uniform per workgroup, before any barrier
 
   extern __shared__ __align__(16) unsigned char smem[];
