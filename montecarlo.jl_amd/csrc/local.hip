@@ -436,6 +436,8 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
 
         const double gt = (W <= 16 || t < 16) ? lo[t & 15] : hi[t & 15];   // G_f[r][t]
         double* Vr = Vw + (size_t)(fb * W + m) * W;                        // raw row t
+        const double uraw = (r == t ? 1.0 : 0.0) - gt;                     // u_t(r), unscaled
+        if (fok && r < W) UwT[(size_t)(f * W + r) * UT + m] = uraw;         // u^T: [f][r][flip]
         if (fok && r == t) {
 #pragma unroll
           for (int c = 0; c < 16; c += 2) {
@@ -443,14 +445,15 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
             if constexpr (W > 16) *reinterpret_cast<d2*>(Vr + 16 + c) = d2{hi[c], hi[c + 1]};
           }
         }
-        const double uraw = (r == t ? 1.0 : 0.0) - gt;                     // u_t(r), unscaled
-        if (fok && r < W) UwT[(size_t)(f * W + r) * UT + m] = uraw;         // u^T: [f][r][flip]
+        // this lane's block: gamma = Delta_f / R_f, one division issued after the stores and
+        // before the wait on them, so that its latency overlaps theirs
+        __builtin_amdgcn_sched_barrier(0);
+        const double gamL = gamma_of(f ? q.D1 : q.D0, f ? q.R1 : q.R0);
+        __builtin_amdgcn_sched_barrier(0);
         site_r = lane == m ? t : site_r;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // this lane's block: gamma = Delta_f / R_f (one division, overlapping the row reads)
-        const double gamL = gamma_of(f ? q.D1 : q.D0, f ? q.R1 : q.R0);
         if (fok && r == m) gam[f * W + m] = gamL;
         const double ug = fok ? gamL * uraw : 0.0;
 #pragma unroll
