@@ -395,6 +395,8 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       const double* Bf = Bn + (size_t)fb * W * LD;
       // every read in flight at once: in-range (clamped) addresses, masked by an exact
       // multiplication with 0 or 1 (the block is finite), no per-column branch
+      // every read in flight at once: in-range (clamped) addresses, masked by an exact
+      // multiplication with 0 or 1 (the block is finite), no per-column branch
       rowvec lo, hi;
 #pragma unroll
       for (int c = 0; c < 16; c++) {
@@ -408,32 +410,54 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
         }
       }
       double dg = Bf[rc * LD + rc] * (rok ? 1.0 : 0.0);
+      // the other flavour block's diagonal entry of this row, updated with the same
+      // operands as its owner's copy, so that every lane holds both of its site's entries
+      const int fo = FLV > 1 ? 1 - fb : fb;
+      double dgo = 0.0;
+      if constexpr (FLV > 1) dgo = Bn[(size_t)fo * W * LD + rc * LD + rc] * (rok ? 1.0 : 0.0);
+      // site r's field value and couplings (lanes r and 32 + r)
+      const int cfr = __shfl(cfl, r, 64);
+      const double dfr = cfr > 0 ? df1 : df0, dsr = cfr > 0 ? ds1 : ds0, bwr = cfr > 0 ? bw1 : bw0;
       // order the reads of Bn before the first flip's stores over it
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_s_waitcnt(0xC07F);
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (int t = 0; t < Wn; t++) {
-        const int x = __builtin_amdgcn_readlane(cfl, t);
-        const int ix = x > 0 ? 1 : 0;
-        const double g0d = readlane_f64(dg, t);
-        const double g1d = FLV > 1 ? readlane_f64(dg, 32 + t) : 0.0;
-        const Proposal q = propose(FLV, g0d, g1d, ix ? df1 : df0, ix ? ds1 : ds0, ix ? bw1 : bw0);
-        const double p = q.p;
-        bool accept;
-        neg += p < 0.0;
-        if (p > 1.0) {
-          accept = true;
-          ties += (p - 1.0 < 1e-12);
-        } else {
-          const double u = readlane_f64(ul, used);
-          used++;
-          accept = u < p;
-          ties += (fabs(u - p) < 1e-12);
-        }
-        p_r = lane == t ? p : p_r;
-        acc_r = lane == t ? (int)accept : acc_r;
-        if (!accept) continue;
+      // Rounds of decisions: every undecided site's proposal is formed at once (lane r: site
+      // r) from the current block; sites before the first accepted one are rejected with
+      // exactly those values (no flip has changed G since their turn), the first accepted
+      // one is flipped, and the next round starts after it.  Uniforms go to the proposals
+      // with p <= 1 in site order, so site r's is the next unused one plus the count of
+      // such sites between the round's start and r.
+      const unsigned int below = (1u << r) - 1u;
+      unsigned int und = Wn >= 32 ? 0xffffffffu : ((1u << Wn) - 1u);
+      double u_r = 0.0;                // uniform drawn for site r (p <= 1)
+      while (und) {
+        const double g0 = f ? dgo : dg, g1 = f ? dg : dgo;
+        const Proposal q = propose(FLV, g0, g1, dfr, dsr, bwr);
+        const bool big = q.p > 1.0;
+        const unsigned int needu = (unsigned int)__ballot(!big) & und;
+        const int idx = used + __popc(needu & below);
+        const double u = __shfl(ul, idx < 64 ? idx : 63, 64);
+        // the flip's division for every site of the round, lane 32 f + r: block f's gamma
+        // of site r, while the uniforms are on their way
+        double gq = gamma_of(f ? q.D1 : q.D0, f ? q.R1 : q.R0);
+        asm volatile("" : "+v"(gq));
+        const bool accept = big || u < q.p;
+        const unsigned int accm = (unsigned int)__ballot(accept) & und;
+        const int ta = accm ? __builtin_ctz(accm) : 32;
+        const unsigned int dec = accm ? (und & ((2u << ta) - 1u)) : und;   // decided this round
+        const bool mine = f == 0 && ((dec >> r) & 1u);
+        p_r = mine ? q.p : p_r;
+        u_r = mine ? u : u_r;
+        acc_r = mine ? (r == ta ? 1 : 0) : acc_r;
+        used += __popc(needu & dec);
+        und &= ~dec;
+        if (!accm) break;
 
+        const int t = ta;
+        const double gam0t = readlane_f64(gq, t);
+        const double gam1t = FLV > 1 ? readlane_f64(gq, 32 + t) : 0.0;
+        const double gamL = f ? gam1t : gam0t;                    // this lane's block
         const double gt = (W <= 16 || t < 16) ? lo[t & 15] : hi[t & 15];   // G_f[r][t]
         double* Vr = Vw + (size_t)(fb * W + m) * W;                        // raw row t
         const double uraw = (r == t ? 1.0 : 0.0) - gt;                     // u_t(r), unscaled
@@ -445,17 +469,18 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
             if constexpr (W > 16) *reinterpret_cast<d2*>(Vr + 16 + c) = d2{hi[c], hi[c + 1]};
           }
         }
-        // this lane's block: gamma = Delta_f / R_f, one division issued after the stores and
-        // before the wait on them, so that its latency overlaps theirs
-        __builtin_amdgcn_sched_barrier(0);
-        const double gamL = gamma_of(f ? q.D1 : q.D0, f ? q.R1 : q.R0);
-        __builtin_amdgcn_sched_barrier(0);
         site_r = lane == m ? t : site_r;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (fok && r == m) gam[f * W + m] = gamL;
+        const int rw = r < W ? r : W - 1;
         const double ug = fok ? gamL * uraw : 0.0;
+        double vo = 0.0, uo = 0.0;
+        if constexpr (FLV > 1) {
+          vo = Vw[(size_t)(fo * W + m) * W + rw];                // G_{1-f}[t][r]
+          uo = UwT[(size_t)(fo * W + rw) * UT + m];             // u_t(r) of the other block
+        }
 #pragma unroll
         for (int c = 0; c < 16; c += 2) {
           const d2 vl = *reinterpret_cast<const d2*>(Vr + c);
@@ -467,7 +492,11 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
             hi[c + 1] = fma(-ug, vh[1], hi[c + 1]);
           }
         }
-        dg = fma(-ug, Vr[r < W ? r : W - 1], dg);
+        dg = fma(-ug, Vr[rw], dg);
+        if constexpr (FLV > 1) {
+          const double ugo = (f ? gam0t : gam1t) * uo;           // the owner's ug, bit for bit
+          dgo = fma(-ugo, vo, dgo);
+        }
         if (lane == 0) sites[m] = t;
         m++;
         // flip m - 1 (u, row, gamma, site; written by several lanes) is complete: publish it
@@ -477,6 +506,13 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0) __hip_atomic_store(flags + 1, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      {
+        // negative probabilities and ties of the window's decided sites
+        const unsigned int win = Wn >= 32 ? 0xffffffffu : ((1u << Wn) - 1u);
+        const bool tie = p_r > 1.0 ? (p_r - 1.0 < 1e-12) : (fabs(u_r - p_r) < 1e-12);
+        neg = __popc((unsigned int)__ballot(f == 0 && p_r < 0.0) & win);
+        ties = __popc((unsigned int)__ballot(f == 0 && tie) & win);
+      }
       const unsigned long long am = __ballot(acc_r != 0);    // whole wave active here
       if (lane == 0) {
         shm[0] = (unsigned long long)m;
@@ -524,18 +560,28 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
         const int sk = sites[k];
         const double gk = gam[fb * W + k];
         const int kk = k & 15;
+        // Terms l >= k carry an unset (zero) x_l or gamma_l and leave the sums unchanged, so
+        // only the 8-wide chunks that start below k are read and summed.
         double cf[W];
         if (wave == 1) {
           // L[k][l] = u_l(s_k): row s_k of u^T, contiguous
-          const double* ur = UwT + (size_t)(fb * W + sk) * UT;    // zero beyond flip k - 1
+          const double* ur = UwT + (size_t)(fb * W + sk) * UT;
 #pragma unroll
-          for (int l = 0; l < W; l++) cf[l] = ur[l];
+          for (int c8 = 0; c8 < W; c8 += 8)
+            if (c8 < k) {
+#pragma unroll
+              for (int l = c8; l < c8 + 8; l++) cf[l] = ur[l];
+            }
           double s4[4] = {r == k ? 1.0 : 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int l = 0; l < W; l++) {
-            const double xl = l < 16 ? fx_lo[l & 15] : fx_hi[l & 15];
-            s4[l & 3] = fma(-cf[l], xl, s4[l & 3]);
-          }
+          for (int c8 = 0; c8 < W; c8 += 8)
+            if (c8 < k) {
+#pragma unroll
+              for (int l = c8; l < c8 + 8; l++) {
+                const double xl = l < 16 ? fx_lo[l & 15] : fx_hi[l & 15];
+                s4[l & 3] = fma(-cf[l], xl, s4[l & 3]);
+              }
+            }
           const double xk = gk * ((s4[0] + s4[1]) + (s4[2] + s4[3]));
           const double olo = fx_lo[kk], ohi = fx_hi[kk];
           fx_lo[kk] = k < 16 ? xk : olo;
@@ -543,17 +589,25 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
         } else {
           // N[i][k] / gamma_i = G_i[s_i][s_k]: column s_k of the published rows
 #pragma unroll
-          for (int i = 0; i < W; i++) cf[i] = Vw[(size_t)(fb * W + i) * W + sk];
+          for (int c8 = 0; c8 < W; c8 += 8)
+            if (c8 < k) {
+#pragma unroll
+              for (int i = c8; i < c8 + 8; i++) cf[i] = Vw[(size_t)(fb * W + i) * W + sk];
+            }
           const double glo = gc_lo[kk], ghi = gc_hi[kk];
           gc_lo[kk] = k < 16 ? gk : glo;
           gc_hi[kk] = k < 16 ? ghi : gk;
           double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int i = 0; i < W; i++) {
-            const double gi = i < 16 ? gc_lo[i & 15] : gc_hi[i & 15];
-            const double bi = i < 16 ? fx_lo[i & 15] : fx_hi[i & 15];
-            s4[i & 3] = fma(bi, gi * cf[i], s4[i & 3]);          // zeros beyond flip k - 1
-          }
+          for (int c8 = 0; c8 < W; c8 += 8)
+            if (c8 < k) {
+#pragma unroll
+              for (int i = c8; i < c8 + 8; i++) {
+                const double gi = i < 16 ? gc_lo[i & 15] : gc_hi[i & 15];
+                const double bi = i < 16 ? fx_lo[i & 15] : fx_hi[i & 15];
+                s4[i & 3] = fma(bi, gi * cf[i], s4[i & 3]);
+              }
+            }
           const double yk = r == k ? 1.0 : (s4[0] + s4[1]) + (s4[2] + s4[3]);
           const double olo = fx_lo[kk], ohi = fx_hi[kk];
           fx_lo[kk] = k < 16 ? yk : olo;
