@@ -105,7 +105,7 @@ def udt_panels(n):
     return out
 
 
-def kernel_work(kind, n, flv, win, per_launch_accepted=0.0):
+def kernel_work(kind, n, flv, win, per_launch_accepted=0.0, windows_per_launch=1):
     """Algorithmic FLOP per launch of each kernel kind at block size n with flv blocks.
 
     gemm: one n^3 product per block.  udt_update: block-reflector updates, 4 rows w cols
@@ -137,23 +137,30 @@ def kernel_work(kind, n, flv, win, per_launch_accepted=0.0):
         m = per_launch_accepted
         return 2.0 * flv * (n * n * m + m * m * n)
     if kind == "local":
-        # window decisions, plus the previous window's update in all but the first launch
-        m = per_launch_accepted
+        # window decisions and window updates of G: a launch per window carries its own
+        # window and the previous window's update (all but the first of a slice); a launch
+        # per slice carries the slice's windows and every update
+        m = per_launch_accepted / windows_per_launch
         own = flv * (2.0 * win * win * m + (2.0 / 3.0) * m ** 3)
-        return own + (nw - 1) / nw * 2.0 * flv * (n * n * m + m * m * n)
+        upd = 2.0 * flv * (n * n * m + m * m * n)
+        if windows_per_launch == 1:
+            return own + (nw - 1) / nw * upd
+        return windows_per_launch * (own + upd)
     return 0.0
 
 
-def kernel_bytes(kind, n, flv, win):
-    """Algorithmic HBM bytes per launch of the window kernel: the window block, the strips
-    G[w, wp], G[wp, w] and the previous factor read, the factor written, field column and
-    log; plus, in all but the first launch of a slice, the previous window's update of G
-    (G read once, written once)."""
+def kernel_bytes(kind, n, flv, win, windows_per_launch=1):
+    """Algorithmic HBM bytes per launch of the window kernel: per window the window block,
+    the strips G[w, wp], G[wp, w] and the previous factor read, the factor written, field
+    column and log; per window update G read once and written once (in all but the first
+    window launch of a slice, or every window of a slice launch)."""
     if kind == "local":
         nw = -(-n // win)
         own = 4.0 * flv * win * win * 8 + flv * win * win * 8 + 3 * win + 4
         upd = 2.0 * flv * n * n * 8
-        return own + (nw - 1) / nw * upd
+        if windows_per_launch == 1:
+            return own + (nw - 1) / nw * upd
+        return windows_per_launch * (own + upd)
     return 0.0
 
 
@@ -460,12 +467,15 @@ def main():
     ch.set_kernel_timing(False)
     kernels = {}
     win = 32                          # the library's default window (delay_depth 0)
-    windows = kt["local"][1]
+    nwin = -(-n // win)
+    launches = kt["local"][1]
+    # one launch per slice (the default) or one per window
+    wpl = nwin if launches and round(launches / ksteps) == 2 * par.slices else 1
     for kind, (ms, cnt) in kt.items():
         if cnt == 0:
             continue
-        apl = acc_k / windows if kind in ("local", "local_apply") and windows else 0.0
-        fl = kernel_work(kind, n, flv, win, apl)
+        apl = acc_k / launches if kind in ("local", "local_apply") and launches else 0.0
+        fl = kernel_work(kind, n, flv, win, apl, wpl if kind == "local" else 1)
         avg_s = ms / cnt / 1e3
         kernels[kind] = {"launches_per_sweep": cnt / ksteps, "avg_us": avg_s * 1e6,
                          "ms_per_sweep": ms / ksteps, "flop_per_launch": fl,
@@ -474,16 +484,18 @@ def main():
     if dom == "local":
         # the window kernel is a latency chain of Metropolis decisions over a block held in
         # registers: its bound is memory (HBM bytes in and out per launch)
-        nbytes = kernel_bytes("local", n, flv, win)
+        nbytes = kernel_bytes("local", n, flv, win, wpl)
+        kname = "window_slice_kernel" if wpl > 1 else "window_kernel"
         achieved = nbytes / (kernels[dom]["avg_us"] * 1e-6) / 1e9
-        roofline = {"bound": "hbm", "kernel": "window_kernel", "achieved": achieved,
+        roofline = {"bound": "hbm", "kernel": kname, "achieved": achieved,
                     "peak": PEAK["hbm_gbs"], "unit": "GB/s", "frac": achieved / PEAK["hbm_gbs"],
-                    "traffic": measured_traffic("window_kernel"),
+                    "traffic": measured_traffic(kname),
                     "avg_launch_us": kernels[dom]["avg_us"], "bytes_per_launch": nbytes,
+                    "windows_per_launch": wpl,
                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/dqmc_sweep_config_b_hbm_traffic.md"}
         # the HIP events around every launch add their own time: the committed rocprofv3
         # summary of the same workload gives the kernel's own duration
-        rp = rocprof_avg_us("window_kernel<double, 2, 32>")
+        rp = rocprof_avg_us(f"{kname}<double, {flv}, {win}>")
         if rp:
             roofline["avg_launch_us_rocprof"] = rp
             roofline["events_over_rocprof"] = kernels[dom]["avg_us"] / rp

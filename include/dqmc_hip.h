@@ -273,6 +273,12 @@ int dqmc_set_wait_limit(dqmc_chain_t h, int64_t polls);
  * is unchanged, recapturing otherwise.  enable = 0 launches kernel by kernel.  Results are
  * the same either way.  No reference counterpart (the reference's sweep is host code). */
 int dqmc_set_sweep_graph(dqmc_chain_t h, int enable);
+/* FP64 chains with windows of <= 32 sites run each slice's local updates
+ * (sweep_spatial!, local_updates.jl:23-60) as one launch: the windows back to back in one
+ * workgroup, each window's update of G applied beside the next window's decisions, ordered by
+ * bounded device-side waits (dqmc_set_wait_limit).  enable = 0 launches once per window.
+ * Results are bit-identical either way.  No reference counterpart (launch scheduling). */
+int dqmc_set_slice_launch(dqmc_chain_t h, int enable);
 /* max|G_wrapped - G_scratch| at every stabilisation since the last call (mixed precision:
  * FP32-propagated G against the FP64 recompute).                                       */
 int dqmc_get_stabilization_log(dqmc_chain_t h, double* out, int64_t capacity, int64_t* length);

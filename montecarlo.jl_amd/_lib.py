@@ -57,7 +57,7 @@ EXPORTED = [
     "dqmc_sweep_spatial", "dqmc_sweep", "dqmc_batch_sweep", "dqmc_measure_equal_time", "dqmc_get_greens", "dqmc_set_greens",
     "dqmc_calculate_greens", "dqmc_unequal_time_greens", "dqmc_combined_greens",
     "dqmc_density_susceptibility", "dqmc_global_update", "dqmc_get_state", "dqmc_get_stats",
-    "dqmc_set_wait_limit", "dqmc_set_sweep_graph",
+    "dqmc_set_wait_limit", "dqmc_set_sweep_graph", "dqmc_set_slice_launch",
     "dqmc_get_stabilization_log", "dqmc_synchronize", "dqmc_la_udt", "dqmc_la_rdivp",
     "dqmc_la_calc_greens_udt", "dqmc_la_gemm", "dqmc_set_kernel_timing", "dqmc_get_kernel_times",
     "dqmc_server_create", "dqmc_server_submit", "dqmc_server_poll", "dqmc_server_wait",
@@ -116,6 +116,7 @@ def load():
         "dqmc_get_stats": [vp, C.POINTER(Stats)],
         "dqmc_set_wait_limit": [vp, i64],
         "dqmc_set_sweep_graph": [vp, i],
+        "dqmc_set_slice_launch": [vp, i],
         "dqmc_get_stabilization_log": [vp, vp, i64, C.POINTER(i64)],
         "dqmc_synchronize": [vp],
         "dqmc_la_udt": [i, _dp, i, _dp, _dp, _dp, _i32],
@@ -462,6 +463,11 @@ class HIPChain:
     def set_sweep_graph(self, enable: bool):
         """Replay sweeps from a captured HIP graph (default) or launch kernel by kernel."""
         self._c(load().dqmc_set_sweep_graph(self._h, int(enable)), "dqmc_set_sweep_graph")
+
+    def set_slice_launch(self, enable: bool):
+        """Run each slice's windows in one launch (default) or launch once per window; the
+        results are bit-identical."""
+        self._c(load().dqmc_set_slice_launch(self._h, int(enable)), "dqmc_set_slice_launch")
 
     def stabilization_log(self) -> np.ndarray:
         length = C.c_int64()

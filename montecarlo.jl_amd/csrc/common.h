@@ -256,6 +256,21 @@ struct WindowArgs {
 };
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s);
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s);
+// One launch per slice (one chain, W in {16, 32}): workgroup 0 runs the slice's windows one
+// after the other, workgroups 1..napply apply each window's update to G as soon as its factor
+// is published.  Window k reads G from B[(k - 1) % 2] (B[0] for k = 0) and apply(k) writes
+// B[(k + 1) % 2], so the slice ends in B[nw % 2].  sync holds 1 + 2 nw counters (zero when
+// the chain is created or rebuilt; every launch advances them, so replays need no reset).
+template <typename GT>
+struct WindowSlice {
+  WindowArgs<GT> a;            // the slice's fields; G, Gd, w0, wp, Sp, Yp, S, Y are per window
+  GT* B[2];                    // the two copies of G
+  GT* Y;                       // two factor slots of flv x W x W
+  int* S;                      // two flip-count slots of W + 1
+  unsigned long long* sync;    // [0] launch epoch, [1 + k] window k published, [1 + nw + k] applications done
+  int nw, napply;
+};
+hipError_t launch_window_slice_f64(const WindowSlice<double>& s, hipStream_t st);
 // Many-chain window launch: `count` chains' WindowArgs in device memory (the same W for all,
 // each with its own n and flavour count).  Workgroup (0, c) runs chain c's window when its
 // do_window is set; workgroups (1 + t, c) apply chain c's previous window to tile t when its

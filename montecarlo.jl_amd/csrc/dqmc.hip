@@ -96,6 +96,8 @@ struct dqmc_chain_s {
   double* errbuf = nullptr;
   long long errcap = 4096, nerr = 0;
   double* scal = nullptr;  // small device scalars
+  unsigned long long* wsync = nullptr;   // one-launch-per-slice counters (1 + 2 ceil(n / 16))
+  bool win_slice = true;   // FP64 sweeps: one window launch per slice (W <= 32)
   long long wait_limit = 1ll << 22;  // polls of the window kernel's factor waves before giving up
   long long dev_err_seen = 0;        // device errors already reported
   double mixed_tol = 1e-6;           // mixed precision: bound on max|G_wrapped - G_scratch|
@@ -800,7 +802,15 @@ int propagate(Chain* h) {
   return wrap_current(h, h->cs + 1 - 1, -1);
 }
 
+// The slice launches' counters start again from zero (a wait that gave up leaves them short).
+int reset_window_sync(Chain* h) {
+  if (h->rec || !h->wsync) return DQMC_OK;
+  HIPCHK(h, launch_fill(reinterpret_cast<double*>(h->wsync), 0.0, 1 + 2 * (int64_t)((h->n + 15) / 16), h->st));
+  return DQMC_OK;
+}
+
 int reverse_build(Chain* h) {
+  CHK(reset_window_sync(h));
   CHK(slot_identity(h, h->K + 1));
   for (int i = h->K; i >= 1; i--) CHK(add_slice_sequence_right(h, i));
   h->cs = 0; h->cr = 1; h->dir = 1;
@@ -811,6 +821,7 @@ int reverse_build(Chain* h) {
 }
 
 int forward_build(Chain* h) {
+  CHK(reset_window_sync(h));
   CHK(slot_identity(h, 1));
   for (int i = 1; i <= h->K; i++) CHK(add_slice_sequence_left(h, i));
   h->cs = h->M + 1; h->cr = h->K; h->dir = -1;
@@ -838,6 +849,29 @@ int sweep_spatial_t(Chain* h, GT** Gp, GT** Gpp, GT* Y) {
   a.uniforms = h->unif; a.n_uniforms = h->nunif; a.seed = h->seed;
   a.ctr = h->ctr; a.dec_log = h->dlog; a.p_log = h->plog; a.log_cap = h->logcap;
   a.wait_limit = h->wait_limit;
+  if constexpr (sizeof(GT) == 8) {
+    if (!h->rec && h->win_slice && W <= 32 && h->wsync) {
+      // one launch for the slice: windows back to back, applications beside them
+      WindowSlice<double> sl;
+      sl.a = a;
+      sl.a.G = src; sl.a.Gd = dst; sl.a.w0 = 0; sl.a.wp = -W; sl.a.do_window = 1;
+      sl.a.Sp = nullptr; sl.a.Yp = nullptr; sl.a.S = nullptr; sl.a.Y = nullptr;
+      sl.B[0] = src; sl.B[1] = dst;
+      sl.Y = Y; sl.S = h->wS;
+      sl.sync = h->wsync;
+      sl.nw = (n + W - 1) / W;
+      const int nt = (n + 63) / 64;
+      sl.napply = std::min(h->flv * nt * nt, 128);
+      {
+        Timed t(h, DQMC_KERNEL_LOCAL);
+        HIPCHK(h, launch_window_slice_f64(sl, h->st));
+      }
+      if (sl.nw & 1) std::swap(src, dst);
+      *Gp = src;
+      *Gpp = dst;
+      return DQMC_OK;
+    }
+  }
   int slot = 0;
   auto launch = [&](int kind) -> int {
     if constexpr (sizeof(GT) == 8) {
@@ -1920,6 +1954,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   A(&h->ctr, 1);
   A(&h->errbuf, (size_t)h->errcap);
   A(&h->scal, 16);
+  A(&h->wsync, 1 + 2 * (size_t)((n + 15) / 16));
   return r;
   });
   if (r != DQMC_OK) {
@@ -1938,6 +1973,7 @@ int dqmc_create(const dqmc_config_t* cfg, dqmc_chain_t* out) {
   ChainCounters z;
   std::memset(&z, 0, sizeof(z));
   e = hipMemcpy(h->ctr, &z, sizeof(z), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(h->wsync, 0, sizeof(unsigned long long) * (1 + 2 * (size_t)((n + 15) / 16)));
   std::vector<int8_t> plus(n * (size_t)h->M, 1);
   if (e == hipSuccess) e = hipMemcpy(h->conf, plus.data(), plus.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -2273,6 +2309,13 @@ int dqmc_set_sweep_graph(dqmc_chain_t h, int enable) {
   if (!h) return DQMC_E_INVALID;
   h->use_graph = enable != 0;
   if (!h->use_graph) drop_sweep_graph(h);
+  return DQMC_OK;
+}
+
+int dqmc_set_slice_launch(dqmc_chain_t h, int enable) {
+  if (!h) return DQMC_E_INVALID;
+  if (h->win_slice != (enable != 0)) drop_sweep_graph(h);
+  h->win_slice = enable != 0;
   return DQMC_OK;
 }
 

@@ -1017,6 +1017,92 @@ __global__ __launch_bounds__(256) void window_apply_kernel(WindowArgs<GT> a) {
   apply_tile<GT, W>(a.Gd, a.G, a.Sp, a.Yp, a.n, a.wp, (int)blockIdx.x, smem);
 }
 
+// Bounded wait until *p >= v (device scope); false when the bound is reached.
+__device__ __forceinline__ bool wait_at_least(const unsigned long long* p, unsigned long long v, long long limit) {
+  long long polls = 0;
+  while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
+    if (++polls > limit) return false;
+    __builtin_amdgcn_s_sleep(2);
+  }
+  return true;
+}
+
+// One launch per slice.  Workgroup 0 runs windows 0 .. nw - 1 back to back: window k waits
+// until every application of window k - 2 is done (its block reads their output copy), runs as
+// in window_kernel, and publishes its factor.  Workgroup 1 + i applies window j to tiles
+// i, i + napply, ... once window j is published and every application of window j - 1 is
+// done (whose source copy window j's application overwrites).  Every wait is bounded; a wait
+// that gives up counts a device error and the launch still ends.  All workgroups are resident
+// at once (one per CU, 1 + napply <= 129), and a workgroup waits only for work that needs no
+// wait of its own to finish first, so the launch always drains.
+template <typename GT, int FLV, int W>
+__global__ __launch_bounds__(256) void window_slice_kernel(WindowSlice<GT> s) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int tid = threadIdx.x;
+  unsigned long long* sync = s.sync;
+  const unsigned long long e = __hip_atomic_load(sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long done = (e + 1) * (unsigned long long)s.napply;
+  unsigned long long* ready = sync + 1;
+  unsigned long long* applied = sync + 1 + s.nw;
+  unsigned long long* errc = reinterpret_cast<unsigned long long*>(&s.a.ctr->device_errors);
+  const size_t ysz = (size_t)FLV * W * W;
+  const long long limit = s.a.wait_limit;
+  if (blockIdx.x == 0) {
+    for (int k = 0; k < s.nw; k++) {
+      if (k >= 2 && tid == 0) {
+        if (!wait_at_least(applied + (k - 2), done, limit)) atomicAdd(errc, 1ull);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      __syncthreads();
+      WindowArgs<GT> a = s.a;
+      a.w0 = k * W;
+      a.wp = a.w0 - W;
+      a.do_window = 1;
+      a.G = k ? s.B[(k - 1) & 1] : s.B[0];
+      a.Gd = k ? s.B[k & 1] : s.B[1];
+      a.Sp = k ? s.S + ((k - 1) & 1) * (W + 1) : nullptr;
+      a.Yp = k ? s.Y + ((k - 1) & 1) * ysz : nullptr;
+      a.S = s.S + (k & 1) * (W + 1);
+      a.Y = s.Y + (k & 1) * ysz;
+      window_role<GT, FLV, W>(a, smem);
+      __builtin_amdgcn_s_waitcnt(0x0F70);     // this wave's global stores have completed
+      __syncthreads();
+      if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(ready + k, e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if (tid == 0) {
+      // every workgroup has read the epoch once the last window's applications are done
+      if (!wait_at_least(applied + (s.nw - 1), done, limit)) atomicAdd(errc, 1ull);
+      __hip_atomic_store(sync, e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return;
+  }
+  const int nt = (s.a.n + 63) >> 6;
+  const int tiles = FLV * nt * nt;
+  for (int j = 0; j < s.nw; j++) {
+    if (tid == 0) {
+      bool ok = wait_at_least(ready + j, e + 1, limit);
+      if (ok && j >= 1) ok = wait_at_least(applied + (j - 1), done, limit);
+      if (!ok) atomicAdd(errc, 1ull);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    for (int t = (int)blockIdx.x - 1; t < tiles; t += s.napply) {
+      apply_tile<GT, W>(s.B[(j + 1) & 1], s.B[j & 1], s.S + (j & 1) * (W + 1), s.Y + (j & 1) * ysz, s.a.n,
+                        j * W, t, smem);
+      __syncthreads();
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(applied + j, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 template <typename GT, int W>
 constexpr size_t apply_lds() {
   return sizeof(GT) * ((size_t)W * W + W * 64 + W * 65 + 64 * (W + 1));
@@ -1102,6 +1188,30 @@ static hipError_t launch_window_t(const WindowArgs<GT>& a, hipStream_t s) {
   return hipGetLastError();
 }
 hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s) { return launch_window_t(a, s); }
+
+template <int FLV, int W>
+static void launch_slice_w(const WindowSlice<double>& s, hipStream_t st) {
+  const size_t wl = window_lds<double, FLV, W>(), al = apply_lds<double, W>();
+  hipLaunchKernelGGL((window_slice_kernel<double, FLV, W>), dim3(1 + s.napply), dim3(256), wl > al ? wl : al, st, s);
+}
+
+hipError_t launch_window_slice_f64(const WindowSlice<double>& s, hipStream_t st) {
+  const WindowArgs<double>& a = s.a;
+  if (a.flv < 1 || a.flv > 2 || (a.W != 16 && a.W != 32) || a.n < 1) return hipErrorInvalidValue;
+  const int nt = (a.n + 63) / 64;
+  if (s.nw != (a.n + a.W - 1) / a.W || s.napply < 1 || s.napply > 128 || s.napply > a.flv * nt * nt)
+    return hipErrorInvalidValue;
+  if (!s.B[0] || !s.B[1] || s.B[0] == s.B[1] || !s.Y || !s.S || !s.sync || !a.ctr || !a.conf)
+    return hipErrorInvalidValue;
+  if (a.flv == 1) {
+    if (a.W == 16) launch_slice_w<1, 16>(s, st);
+    else launch_slice_w<1, 32>(s, st);
+  } else {
+    if (a.W == 16) launch_slice_w<2, 16>(s, st);
+    else launch_slice_w<2, 32>(s, st);
+  }
+  return hipGetLastError();
+}
 hipError_t launch_window_f32(const WindowArgs<float>& a, hipStream_t s) { return launch_window_t(a, s); }
 
 hipError_t local_kernels_init() {
@@ -1125,7 +1235,11 @@ hipError_t local_kernels_init() {
                        reinterpret_cast<const void*>(window_apply_kernel<float, 64>),
                        reinterpret_cast<const void*>(window_multi_kernel<double, 16>),
                        reinterpret_cast<const void*>(window_multi_kernel<double, 32>),
-                       reinterpret_cast<const void*>(window_multi_kernel<double, 64>)};
+                       reinterpret_cast<const void*>(window_multi_kernel<double, 64>),
+                       reinterpret_cast<const void*>(window_slice_kernel<double, 1, 16>),
+                       reinterpret_cast<const void*>(window_slice_kernel<double, 1, 32>),
+                       reinterpret_cast<const void*>(window_slice_kernel<double, 2, 16>),
+                       reinterpret_cast<const void*>(window_slice_kernel<double, 2, 32>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
