@@ -115,6 +115,28 @@ constexpr size_t window_lds() {
   return (loop > block ? loop : block) + 16;     // + the two progress flags (last 16 bytes)
 }
 
+// Pipelining of the one-launch-per-slice form (workgroup 0, W <= 32): the fourth wave, idle
+// during the decisions, prefetches the next window's three block operands into LDS; the
+// window's factor is also kept in LDS for the next window's block; and the fourth wave
+// publishes the previous window's factor once the current window's block is formed.
+template <typename GT>
+struct SliceCtl {
+  GT* pf;                              // [3 FLV][W][LD]: G[w',w'], G[w',w], G[w,w'] of the next
+                                       // window w' (the staging layout of mats 0-2)
+  GT* yl;                              // [FLV][W][LD]: the window's factor (staging layout of mat 3)
+  bool pf_in;                          // this window's operands: pf and yl (else global memory)
+  const GT* gnext;                     // copy that holds the next window's operands (nullptr: none)
+  const unsigned long long* wait_p;    // gnext is complete once *wait_p >= wait_v (nullptr: already)
+  unsigned long long wait_v;
+  unsigned long long* pub_p;           // previous window: *pub_p = pub_v once its stores are done
+  unsigned long long pub_v;
+  unsigned long long* errc;            // device-error count of a wait that gives up
+  long long limit;
+};
+
+template <typename GT, int FLV, int W>
+constexpr size_t slice_extra_lds() { return (size_t)4 * FLV * W * (W + 1) * sizeof(GT); }
+
 // The window's block of G after the previous window's flips, from the source copy:
 //   B = G[w, w] - C[w, :] Y R[:, w] = G[w, w] + G[w, wp] (Y G[wp, w])
 // (C = (I - G)[:, wp] has no identity part in the rows of this window), on the matrix
@@ -212,7 +234,8 @@ __device__ __forceinline__ void window_block_direct(const WindowArgs<GT>& a, dou
 
 template <typename GT, int FLV, int W>
 __device__ __forceinline__ void window_block_staged(const WindowArgs<GT>& a, double* Bn, GT* St, GT* Zw, int Wn,
-                                                    int tid, int lane, int wave) {
+                                                    int tid, int lane, int wave,
+                                                    const SliceCtl<GT>* sc = nullptr) {
   typedef typename V4<GT>::type acc_t;
   constexpr int CT = W / 16, KS = W / 4, LD = W + 1, WW = W * W, QM = WW / 256;
   static_assert(WW % 256 == 0, "staging assumes whole passes of 256 threads per matrix");
@@ -223,6 +246,8 @@ __device__ __forceinline__ void window_block_staged(const WindowArgs<GT>& a, dou
   const int mp = corr ? a.Sp[0] : 0;
   // St[(4 b + mat) W LD + c LD + r]: mat 0 = G[w, w], 1 = G[w, wp], 2 = G[wp, w]
   // (column-major, [c][r]); mat 3 = previous factor (row-major, [p][q])
+  const bool lds_in = sc != nullptr && sc->pf_in;
+  if (!lds_in) {
   GT v[FLV][4][QM];
 #pragma unroll
   for (int b = 0; b < FLV; b++) {
@@ -256,12 +281,13 @@ __device__ __forceinline__ void window_block_staged(const WindowArgs<GT>& a, dou
       }
     }
   __syncthreads();
+  }
   for (int unit = wave; unit < FLV * CT; unit += 4) {
     const int b = unit / CT, ci = unit - b * CT;
-    const GT* Bs = St + (size_t)(4 * b) * W * LD;
+    const GT* Bs = lds_in ? sc->pf + (size_t)(3 * b) * W * LD : St + (size_t)(4 * b) * W * LD;
     const GT* Ps = Bs + W * LD;
     const GT* Qs = Bs + 2 * W * LD;
-    const GT* Ys = Bs + 3 * W * LD;
+    const GT* Ys = lds_in ? sc->yl + (size_t)b * W * LD : Bs + 3 * W * LD;
     const int cc = ci * 16 + li;
     acc_t o[CT];
 #pragma unroll
@@ -314,7 +340,8 @@ __device__ __forceinline__ void window_block_staged(const WindowArgs<GT>& a, dou
 }
 
 template <typename GT, int FLV, int W>
-__device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned char* smem) {
+__device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned char* smem,
+                                            const SliceCtl<GT>* sc = nullptr) {
   constexpr int NW = 4, CPW = W / NW, LD = W + 1;
   typedef double colvec __attribute__((ext_vector_type(CPW)));
   // LDS, per flavour block b: Uw[b][l][r] = u_l(r);  Vw[b][l][vpos(c)] = G_l[s_l][c] (the
@@ -352,7 +379,7 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
   double* Bn = Uw;
   if constexpr (W <= 32) {
     GT* St = reinterpret_cast<GT*>(Uw + FLV * W * LD);
-    window_block_staged<GT, FLV, W>(a, Bn, St, St + 4 * FLV * W * LD, Wn, tid, lane, wave);
+    window_block_staged<GT, FLV, W>(a, Bn, St, St + 4 * FLV * W * LD, Wn, tid, lane, wave, sc);
   } else {
     window_block_direct<GT, FLV, W>(a, Bn, reinterpret_cast<GT*>(Vw), Wn, lane, wave);
   }
@@ -373,7 +400,15 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       Vw[e] = 0.0;
       UwT[e] = 0.0;
     }
+    // the previous window's factor, flip count and field went out before this window
+    // started: every wave's stores are complete at the barrier, and the fourth wave
+    // publishes them while the decisions run
+    if (sc && sc->pub_p) __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
+    if (sc && sc->pub_p && tid == 192) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(sc->pub_p, sc->pub_v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   // factor waves (W <= 32): wave 1 column r of X, wave 2 row r of (I - N)^-1, and gammas
   typedef double fvec __attribute__((ext_vector_type(16)));
@@ -615,6 +650,47 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
         }
         k++;
       }
+    } else if (sc && sc->gnext) {
+      // wave 3: the next window's operands G[w', w'], G[w', w], G[w, w'] from the copy
+      // that the updates of the window before this one write, once they are all done
+      if (sc->wait_p) {
+        bool ok = true;
+        long long polls = 0;
+        while (__hip_atomic_load(sc->wait_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sc->wait_v) {
+          if (++polls > sc->limit) { ok = false; break; }
+          __builtin_amdgcn_s_sleep(2);
+        }
+        if (!ok && lane == 0) atomicAdd(sc->errc, 1ull);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      const int w0n = w0 + W, wpn = w0;
+      const int Wnn = n - w0n < W ? n - w0n : W;
+      constexpr int QL = W * W / 64;
+#pragma unroll
+      for (int mat = 0; mat < 3; mat++) {
+        GT v[FLV][QL];
+#pragma unroll
+        for (int b = 0; b < FLV; b++) {
+          const GT* Gb = sc->gnext + (size_t)b * n * n;
+#pragma unroll
+          for (int q = 0; q < QL; q++) {
+            const int idx = lane + 64 * q;
+            const int rr = idx % W, cc = idx / W;
+            const int rw = w0n + (rr < Wnn ? rr : Wnn - 1), cw = w0n + (cc < Wnn ? cc : Wnn - 1);
+            const int row = mat == 2 ? wpn + rr : rw;
+            const int col = mat == 1 ? wpn + cc : cw;
+            v[b][q] = Gb[(size_t)row + (size_t)col * n];
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < FLV; b++)
+#pragma unroll
+          for (int q = 0; q < QL; q++) {
+            const int idx = lane + 64 * q;
+            const int rr = idx % W, cc = idx / W;
+            sc->pf[(size_t)(3 * b + mat) * W * LD + cc * LD + rr] = v[b][q];
+          }
+      }
     }
     __syncthreads();
     m = (int)shm[0];
@@ -755,7 +831,10 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const int j = ti * 16 + mfma_row<double>(lane, q);
-          if (j < m && c < m) Yb[(size_t)sites[j] * W + sites[c]] = (GT)acc[q];
+          if (j < m && c < m) {
+            Yb[(size_t)sites[j] * W + sites[c]] = (GT)acc[q];
+            if (sc) sc->yl[(size_t)b * W * LD + sites[j] * LD + sites[c]] = (GT)acc[q];
+          }
         }
       }
     }
@@ -863,7 +942,12 @@ __device__ __forceinline__ void window_role(const WindowArgs<GT>& a, unsigned ch
       const int e = tid + 256 * q;
       const int rem = e % (W * W);
       const int pr = rem / W, pc = rem - pr * W;
-      if (!(((accm >> pr) & 1ull) && ((accm >> pc) & 1ull))) a.Y[e] = GT(0);
+      if (!(((accm >> pr) & 1ull) && ((accm >> pc) & 1ull))) {
+        a.Y[e] = GT(0);
+        if constexpr (W <= 32) {
+          if (sc) sc->yl[(size_t)(e / (W * W)) * W * LD + pr * LD + pc] = GT(0);
+        }
+      }
     }
   }
   if (tid == 0) a.S[0] = m;
@@ -1027,9 +1111,10 @@ __device__ __forceinline__ bool wait_at_least(const unsigned long long* p, unsig
   return true;
 }
 
-// One launch per slice.  Workgroup 0 runs windows 0 .. nw - 1 back to back: window k waits
-// until every application of window k - 2 is done (its block reads their output copy), runs as
-// in window_kernel, and publishes its factor.  Workgroup 1 + i applies window j to tiles
+// One launch per slice.  Workgroup 0 runs windows 0 .. nw - 1 back to back, as in
+// window_kernel, pipelined (SliceCtl): window k's block operands were prefetched during window
+// k - 1 once every application of window k - 2 was done (window k reads their output copy),
+// and window k - 1's factor is published once window k's block is formed.  Workgroup 1 + i applies window j to tiles
 // i, i + napply, ... once window j is published and every application of window j - 1 is
 // done (whose source copy window j's application overwrites).  Every wait is bounded; a wait
 // that gives up counts a device error and the launch still ends.  All workgroups are resident
@@ -1048,11 +1133,16 @@ __global__ __launch_bounds__(256) void window_slice_kernel(WindowSlice<GT> s) {
   const size_t ysz = (size_t)FLV * W * W;
   const long long limit = s.a.wait_limit;
   if (blockIdx.x == 0) {
+    // window k's block operands: window 0's from global memory, every later window's
+    // prefetched during the previous window (from the copy that window k - 2's updates
+    // write, after they are done); window k - 1 is published once window k's block is formed
+    GT* pf = reinterpret_cast<GT*>(smem + window_lds<GT, FLV, W>());
+    SliceCtl<GT> sc;
+    sc.pf = pf;
+    sc.yl = pf + (size_t)3 * FLV * W * (W + 1);
+    sc.errc = errc;
+    sc.limit = limit;
     for (int k = 0; k < s.nw; k++) {
-      if (k >= 2 && tid == 0) {
-        if (!wait_at_least(applied + (k - 2), done, limit)) atomicAdd(errc, 1ull);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      }
       __syncthreads();
       WindowArgs<GT> a = s.a;
       a.w0 = k * W;
@@ -1064,13 +1154,19 @@ __global__ __launch_bounds__(256) void window_slice_kernel(WindowSlice<GT> s) {
       a.Yp = k ? s.Y + ((k - 1) & 1) * ysz : nullptr;
       a.S = s.S + (k & 1) * (W + 1);
       a.Y = s.Y + (k & 1) * ysz;
-      window_role<GT, FLV, W>(a, smem);
-      __builtin_amdgcn_s_waitcnt(0x0F70);     // this wave's global stores have completed
-      __syncthreads();
-      if (tid == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_store(ready + k, e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      sc.pf_in = k > 0;
+      sc.gnext = k + 1 < s.nw ? s.B[k & 1] : nullptr;         // window k + 1 reads B[k % 2]
+      sc.wait_p = k >= 1 ? applied + (k - 1) : nullptr;
+      sc.wait_v = done;
+      sc.pub_p = k >= 1 ? ready + (k - 1) : nullptr;
+      sc.pub_v = e + 1;
+      window_role<GT, FLV, W>(a, smem, &sc);
+    }
+    __builtin_amdgcn_s_waitcnt(0x0F70);     // the last window's global stores have completed
+    __syncthreads();
+    if (tid == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_store(ready + (s.nw - 1), e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (tid == 0) {
       // every workgroup has read the epoch once the last window's applications are done
@@ -1191,7 +1287,9 @@ hipError_t launch_window_f64(const WindowArgs<double>& a, hipStream_t s) { retur
 
 template <int FLV, int W>
 static void launch_slice_w(const WindowSlice<double>& s, hipStream_t st) {
-  const size_t wl = window_lds<double, FLV, W>(), al = apply_lds<double, W>();
+  const size_t wl = window_lds<double, FLV, W>() + slice_extra_lds<double, FLV, W>(), al = apply_lds<double, W>();
+  static_assert(window_lds<double, FLV, W>() % 16 == 0, "prefetch area alignment");
+  static_assert(window_lds<double, FLV, W>() + slice_extra_lds<double, FLV, W>() <= 160 * 1024, "LDS");
   hipLaunchKernelGGL((window_slice_kernel<double, FLV, W>), dim3(1 + s.napply), dim3(256), wl > al ? wl : al, st, s);
 }
 
