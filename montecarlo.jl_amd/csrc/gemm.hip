@@ -189,23 +189,29 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmBatch<T> batch) {
 #pragma unroll
     for (int j = 0; j < kF; j++) acc[i][j] = acc_t{0, 0, 0, 0};
   const int li = lane & 15, lk = lane >> 4;
+  auto step = [&](const T* Ab, const T* Bb, int kk) {
+    T av[kF], bv[kF];
+#pragma unroll
+    for (int i = 0; i < kF; i++) av[i] = Ab[(kk + lk) * kLD + wm + 16 * i + li];
+#pragma unroll
+    for (int j = 0; j < kF; j++) bv[j] = Bb[(kk + lk) * kLD + wn + 16 * j + li];
+#pragma unroll
+    for (int i = 0; i < kF; i++)
+#pragma unroll
+      for (int j = 0; j < kF; j++) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
+  };
   auto compute = [&](int c, int buf) {
     const T* Ab = As + buf * kKC * kLD;
     const T* Bb = Bs + buf * kKC * kLD;
     const int kend = K - c * kKC < kKC ? K - c * kKC : kKC;
+    if (kend == kKC) {
+      // a whole chunk: straight-line k-steps, so the LDS reads run ahead of the products
+#pragma unroll
+      for (int kk = 0; kk < kKC; kk += 4) step(Ab, Bb, kk);
+    } else {
 #pragma unroll 8
-    for (int kk = 0; kk < kKC; kk += 4) {
-      if (kk < kend) {
-        T av[kF], bv[kF];
-#pragma unroll
-        for (int i = 0; i < kF; i++) av[i] = Ab[(kk + lk) * kLD + wm + 16 * i + li];
-#pragma unroll
-        for (int j = 0; j < kF; j++) bv[j] = Bb[(kk + lk) * kLD + wn + 16 * j + li];
-#pragma unroll
-        for (int i = 0; i < kF; i++)
-#pragma unroll
-          for (int j = 0; j < kF; j++) acc[i][j] = mfma16(av[i], bv[j], acc[i][j]);
-      }
+      for (int kk = 0; kk < kKC; kk += 4)
+        if (kk < kend) step(Ab, Bb, kk);
     }
   };
 
