@@ -1165,13 +1165,165 @@ __global__ __launch_bounds__(64) void rdivp_strip_kernel(RdivpBatch batch) {
   }
 }
 
+// rdivp for n <= kRdivpBlockMaxN: four waves per 16-row strip.  Every 16 x 16 diagonal block
+// of T is inverted first (thread (b, c) solves U_b x = e_c upwards), beside the loads of
+// the strip's pivoted input; then per 16-column panel p the four waves share the update
+// Y = X_p - X_{<p} T_{<p,p} on the matrix cores (wave w takes k-steps w, w + 4, ...; T's
+// coupling column of the next panel is loaded while the current one is formed), and wave 0
+// forms X_p = Y inv(T_pp), one 16 x 16 x 16 product.  In exact arithmetic this is the
+// column-by-column substitution of the reference; the products round differently (the
+// bound of the kernel tests: 1e-12 of the largest entry).
+constexpr int kRdivpBlockMaxN = 512;
+
+static size_t rdivp_block_lds_bytes(int n) {
+  const size_t nb = (size_t)(n + 15) / 16;
+  return ((size_t)n * 17 + nb * 16 * 17 + 4 * 4 * 64 + 16 * 17) * 8 + (size_t)n * 4;
+}
+
+// KS = 16 (n <= 256) or 32 (n <= 512): loads per thread of the gather and k-steps of one wave
+// per panel, at most.  Every batch of loads has this fixed count (clamped addresses), so the
+// wait for one batch never waits for the prefetch behind it.
+template <bool TAB, int KS>
+__global__ __launch_bounds__(256) void rdivp_block_kernel(RdivpBatch batch) {
+  constexpr int kRdivpBlockSteps = KS;
+  const RdivpItem it = TAB ? batch.items[blockIdx.y] : batch.it[blockIdx.y];
+  const int n = TAB ? it.n : batch.n;
+  const int r0 = blockIdx.x * 16;
+  if (TAB && r0 >= n) return;   // item-table launches size the grid for the largest item
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, li = lane & 15, lk = lane >> 4;
+  const int nb = (n + 15) / 16;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* X = reinterpret_cast<double*>(smem);          // [n][17]: X[j * 17 + row]
+  double* Ti = X + (size_t)n * 17;                      // [nb][16][17]: block b, row a, column c
+  double* red = Ti + (size_t)nb * 16 * 17;              // [4 waves][4][64]
+  double* Ysh = red + 4 * 4 * 64;                       // [16][17]: Y[row][col]
+  int* pv = reinterpret_cast<int*>(Ysh + 16 * 17);
+  double* __restrict__ A = it.A;
+  const double* __restrict__ T = it.T;
+
+  for (int j = tid; j < n; j += 256) pv[j] = it.pivot[j];
+  __syncthreads();
+  // the strip's pivoted input (thread: row tid % 16, columns tid / 16 + 16 q), held in
+  // registers while the diagonal blocks are inverted
+  const int grow = tid & 15, gcol = tid >> 4;
+  const int grr = r0 + grow, grc = grr < n ? grr : n - 1;
+  double gv[kRdivpBlockSteps];
+#pragma unroll
+  for (int q = 0; q < kRdivpBlockSteps; q++) {
+    const int j = gcol + 16 * q;
+    gv[q] = A[(size_t)grc + (size_t)pv[j < n ? j : n - 1] * n];
+  }
+  // diagonal blocks of T: the upper part, the identity beyond n (consecutive threads read
+  // consecutive rows of a column)
+  for (int e = tid; e < nb * 256; e += 256) {
+    const int b = e >> 8, c = (e >> 4) & 15, a = e & 15;
+    const int ra = 16 * b + a, cc = 16 * b + c;
+    double v;
+    if (ra < n && cc < n) v = a <= c ? T[(size_t)ra + (size_t)cc * n] : 0.0;
+    else v = a == c ? 1.0 : 0.0;
+    Ti[(size_t)(b * 16 + a) * 17 + c] = v;
+  }
+  __syncthreads();
+  for (int b0 = 0; b0 < nb; b0 += 16) {
+    const int b = b0 + (tid >> 4), c = tid & 15;
+    double x[16];
+    if (b < nb) {
+      const double* U = Ti + (size_t)b * 16 * 17;
+      // x[k] = 0 for k > c, so every row runs the same straight-line sum
+#pragma unroll
+      for (int i = 15; i >= 0; i--) {
+        double sum = i == c ? 1.0 : 0.0;
+#pragma unroll
+        for (int k = i + 1; k < 16; k++) sum -= U[i * 17 + k] * x[k];
+        x[i] = i <= c ? sum / U[i * 17 + i] : 0.0;
+      }
+    }
+    __syncthreads();
+    if (b < nb) {
+#pragma unroll
+      for (int a = 0; a < 16; a++) Ti[(size_t)(b * 16 + a) * 17 + c] = x[a];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kRdivpBlockSteps; q++) {
+    const int j = gcol + 16 * q;
+    if (j < n) X[(size_t)j * 17 + grow] = gv[q];
+  }
+  __syncthreads();
+
+  // T's coupling column of panel j0 for this wave's k-steps q = wave + 4 s (4 q < j0)
+  auto load_tb = [&](double (&tb)[KS], int j0) {
+    const int cb = j0 + li, cbc = cb < n ? cb : n - 1;
+#pragma unroll
+    for (int s = 0; s < kRdivpBlockSteps; s++) {
+      const int k = 4 * (wave + 4 * s) + lk;
+      tb[s] = T[(size_t)(k < n ? k : n - 1) + (size_t)cbc * n];
+    }
+  };
+  auto panel = [&](int p, const double (&tb)[KS]) {
+    const int j0 = 16 * p;
+    dx4 acc = dx4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kRdivpBlockSteps; s++) {
+      const int q = wave + 4 * s;
+      if (4 * q < j0) acc = mfma16(X[(size_t)(4 * q + lk) * 17 + li], tb[s], acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) red[(wave * 4 + r) * 64 + lane] = acc[r];
+    __syncthreads();
+    if (wave == 0) {
+      const int cb = j0 + li;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = mfma_row<double>(lane, r);
+        const double u = (red[r * 64 + lane] + red[(4 + r) * 64 + lane]) +
+                         (red[(8 + r) * 64 + lane] + red[(12 + r) * 64 + lane]);
+        Ysh[row * 17 + li] = cb < n ? X[(size_t)cb * 17 + row] - u : 0.0;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      dx4 o = dx4{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < 4; ks++) {
+        const int k = 4 * ks + lk;
+        o = mfma16(Ysh[li * 17 + k], Ti[(size_t)(p * 16 + k) * 17 + li], o);
+      }
+      if (cb < n) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) X[(size_t)cb * 17 + mfma_row<double>(lane, r)] = o[r];
+      }
+    }
+    __syncthreads();
+  };
+  double tbA[kRdivpBlockSteps], tbB[kRdivpBlockSteps];
+  load_tb(tbA, 0);
+  for (int p = 0; p < nb; p += 2) {
+    if (p + 1 < nb) load_tb(tbB, 16 * (p + 1));
+    panel(p, tbA);
+    if (p + 1 < nb) {
+      if (p + 2 < nb) load_tb(tbA, 16 * (p + 2));
+      panel(p + 1, tbB);
+    }
+  }
+  if (grr < n) {
+    for (int j = gcol; j < n; j += 16) A[(size_t)grr + (size_t)j * n] = X[(size_t)j * 17 + grow];
+  }
+}
+
 hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
   if (b.count <= 0) return hipSuccess;
   if (b.items) {
     // item table: one kernel class per launch (the host keeps n <= kRdivpStripMaxN items
     // and larger ones apart); the grid and LDS are sized for the largest item
     if (b.max_n < 1 || b.max_n > 2048) return hipErrorInvalidValue;
-    if (b.max_n <= kRdivpStripMaxN)
+    if (b.max_n <= 256)
+      hipLaunchKernelGGL((rdivp_block_kernel<true, 16>), dim3((b.max_n + 15) / 16, b.count), dim3(256),
+                         rdivp_block_lds_bytes(b.max_n), s, b);
+    else if (b.max_n <= kRdivpBlockMaxN)
+      hipLaunchKernelGGL((rdivp_block_kernel<true, 32>), dim3((b.max_n + 15) / 16, b.count), dim3(256),
+                         rdivp_block_lds_bytes(b.max_n), s, b);
+    else if (b.max_n <= kRdivpStripMaxN)
       hipLaunchKernelGGL(rdivp_strip_kernel<true>, dim3((b.max_n + 15) / 16, b.count), dim3(64),
                          rdivp_strip_lds_bytes(b.max_n), s, b);
     else
@@ -1179,7 +1331,13 @@ hipError_t launch_rdivp(const RdivpBatch& b, hipStream_t s) {
     return hipGetLastError();
   }
   if (b.count > kMaxUdtItems) return hipErrorInvalidValue;
-  if (b.n <= kRdivpStripMaxN)
+  if (b.n <= 256)
+    hipLaunchKernelGGL((rdivp_block_kernel<false, 16>), dim3((b.n + 15) / 16, b.count), dim3(256),
+                       rdivp_block_lds_bytes(b.n), s, b);
+  else if (b.n <= kRdivpBlockMaxN)
+    hipLaunchKernelGGL((rdivp_block_kernel<false, 32>), dim3((b.n + 15) / 16, b.count), dim3(256),
+                       rdivp_block_lds_bytes(b.n), s, b);
+  else if (b.n <= kRdivpStripMaxN)
     hipLaunchKernelGGL(rdivp_strip_kernel<false>, dim3((b.n + 15) / 16, b.count), dim3(64),
                        rdivp_strip_lds_bytes(b.n), s, b);
   else
@@ -1197,7 +1355,11 @@ hipError_t qr_kernels_init() {
                        reinterpret_cast<const void*>(udt_panel_kernel<4, 16, true>),
                        reinterpret_cast<const void*>(udt_panel_kernel<8, 8, true>),
                        reinterpret_cast<const void*>(rdivp_strip_kernel<false>),
-                       reinterpret_cast<const void*>(rdivp_strip_kernel<true>)};
+                       reinterpret_cast<const void*>(rdivp_strip_kernel<true>),
+                       reinterpret_cast<const void*>(rdivp_block_kernel<false, 16>),
+                       reinterpret_cast<const void*>(rdivp_block_kernel<true, 16>),
+                       reinterpret_cast<const void*>(rdivp_block_kernel<false, 32>),
+                       reinterpret_cast<const void*>(rdivp_block_kernel<true, 32>)};
   for (const void* f : fns) {
     hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
