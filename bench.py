@@ -466,7 +466,7 @@ def main():
     acc_k = ch.stats()["accepted"] - acc_k0
     ch.set_kernel_timing(False)
     kernels = {}
-    win = 32                          # the library's default window (delay_depth 0)
+    win = 16 if n <= 256 else 32      # the library's default window (window_size, delay_depth 0)
     nwin = -(-n // win)
     launches = kt["local"][1]
     # one launch per slice (the default) or one per window

@@ -1204,12 +1204,14 @@ constexpr size_t apply_lds() {
   return sizeof(GT) * ((size_t)W * W + W * 64 + W * 65 + 64 * (W + 1));
 }
 
-// 32 sites per window: measured fastest at n = 256 (per-flip work and the m x m solves
-// grow with W, launches with 1 / W); 16 and 64 are selectable through delay_depth.
+// Sites per window (delay_depth selects 16, 32 or 64 explicitly).  Per-flip work and the
+// m x m solves grow with W, the per-window block formation and factor build with 1 / W.
+// Measured with one launch per slice: 16 is faster up to n = 256 (config B 71.5 against
+// 73.7 ms per sweep; the 64-chain config-D batch 323 against 313 sweeps/s), 32 beyond
+// (n = 576: 326 against 335 ms; n = 1024: 373 against 406 ms).
 int window_size(int n, int flv) {
-  (void)n;
   (void)flv;
-  return 32;
+  return n <= 256 ? 16 : 32;
 }
 
 template <typename GT, int FLV, int W>

@@ -10,9 +10,9 @@ from conftest import make_chain
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("L,U,beta,W", [(4, 4.0, 2.0, 0), (6, 4.0, 2.0, 0), (12, 4.0, 1.0, 0),
-                                        (16, 4.0, 1.0, 0), (8, -4.0, 2.0, 0), (6, 4.0, 2.0, 16),
-                                        (16, 4.0, 1.0, 16), (24, -4.0, 1.0, 0)])
+@pytest.mark.parametrize("L,U,beta,W", [(4, 4.0, 2.0, 0), (6, 4.0, 2.0, 0), (12, 4.0, 1.0, 32),
+                                        (16, 4.0, 1.0, 0), (8, -4.0, 2.0, 32), (6, 4.0, 2.0, 32),
+                                        (16, 4.0, 1.0, 32), (24, -4.0, 1.0, 0), (24, -4.0, 1.0, 16)])
 def test_slice_launch_equals_window_launches(pkg, hip, L, U, beta, W):
     chains = []
     try:
