@@ -276,7 +276,10 @@ int dqmc_set_sweep_graph(dqmc_chain_t h, int enable);
 /* FP64 chains with windows of <= 32 sites run each slice's local updates
  * (sweep_spatial!, local_updates.jl:23-60) as one launch: the windows back to back in one
  * workgroup, each window's update of G applied beside the next window's decisions, ordered by
- * bounded device-side waits (dqmc_set_wait_limit).  enable = 0 launches once per window.
+ * bounded device-side waits (dqmc_set_wait_limit).  The launch's 1 + min(b (n/64)^2, 128)
+ * workgroups must be resident together (one per CU): chains swept concurrently on other
+ * streams of the same GPU should use enable = 0 or dqmc_batch_sweep, otherwise a wait can
+ * run to its bound (reported as a device error).  enable = 0 launches once per window.
  * Results are bit-identical either way.  No reference counterpart (launch scheduling). */
 int dqmc_set_slice_launch(dqmc_chain_t h, int enable);
 /* max|G_wrapped - G_scratch| at every stabilisation since the last call (mixed precision:
