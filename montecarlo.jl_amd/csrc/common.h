@@ -87,7 +87,7 @@ struct UdtItem {
   double* T;        // D^-1 R, pivoted (apply_pivot) or "dirty upper" (may alias A then)
   int* pivot;       // column permutation, 0-based
   double* tau;      // n scratch
-  double* tfac;     // (n + 16) * 16 scratch (compact-WY T factors, 16 x 16 per panel start)
+  double* tfac;     // (n + 16) * 16 scratch (each panel's 16 taus, at 16 * its start column)
   double* swap;     // 32 * n scratch (column moves)
   unsigned long long* err;   // device error counter (may be null): +1 per broken pivot selection
   // many-chain launches (item table in device memory): the item's own size and, for the

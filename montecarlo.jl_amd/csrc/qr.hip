@@ -63,10 +63,11 @@ __device__ __forceinline__ dx4 mfma16(double a, double b, dx4 c) {
 
 // The UDT runs as a sequence of launches per batch (one stream, so each sees the previous
 // one's results): column norms; per panel a single-workgroup panel kernel (pivot choice,
-// column moves, Householder factorisation of the panel in LDS, compact-WY factor T) and a
-// many-workgroup launch that applies the panel's block reflector to the trailing matrix
-// and accumulates Q forward on U's rows (U <- U H); finally D and T.  Per-item state between launches: it.tau holds the residual
-// norms, it.pivot the working permutation, it.tfac the T factor of each panel.
+// column moves, Householder factorisation of the panel) and a many-workgroup launch that
+// forms the panel's compact-WY factor T, applies the block reflector to the trailing matrix
+// and accumulates Q forward on U's rows (U <- U H); finally D and T.  Per-item state
+// between launches: it.tau holds the residual norms, it.pivot the working permutation,
+// it.tfac the taus of each panel.
 
 // Rows of the panel per thread of the panel kernel (256 threads): 1, 2, 4 or 8.
 
@@ -347,7 +348,7 @@ __device__ __forceinline__ void panel_steps_block(double (&a)[R][WM], double* re
 }
 
 // Householder panel of one item: pivot choice for the panel, column moves, factorisation
-// with the panel rows in registers (R rows x WM columns per thread), compact-WY factor T.
+// with the panel rows in registers (R rows x WM columns per thread), the taus.
 // Per column step one block reduction gives ||x||^2 and x^T a_c for the later columns:
 // with v = (x + nu e_j) / (x_j + nu), v^T a_c = a_jc + (x_{>j}^T a_c{>j}) / (x_j + nu).
 template <int R, int WM, bool TAB>
@@ -563,7 +564,8 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   panel_steps_block<0, R, WM>(a, red, wt, rowj, sc, tid, lane, wave, rows, w);
   __syncthreads();
   }
-  // panel -> A and LDS (V for S = V^T V)
+  // panel -> A, and the panel's taus for the reflector launch, which forms S = V^T V and
+  // the compact-WY factor T from them (sc holds the taus since the factorisation's barrier)
 #pragma unroll
   for (int k = 0; k < RT; k++) {
     const int i = tid + TP * k;
@@ -572,7 +574,6 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
       for (int c = 0; c < WM; c++)
         if (c < w) {
           A[(size_t)(p0 + i) + (size_t)(p0 + c) * n] = a[k][c];
-          P[i + (size_t)rows * c] = a[k][c];
           if (last && c == i) {   // D = |diag R| (0 -> 1) of this panel
             const double x = fabs(a[k][c]);
             it.D[p0 + i] = x == 0.0 ? 1.0 : x;
@@ -580,70 +581,9 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
         }
     }
   }
-
-  // compact-WY factor T (upper triangular, 16 x 16, zero beyond w):
-  // H_1..H_w = I - V T V^T.  S = V^T V on the matrix cores (the four waves take every
-  // fourth k-step, partial sums reduced through LDS), then lane i of wave 0 builds row i
-  // of T from its own earlier entries, T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j]
-  // (T[i][k] = 0 for k < i, so the sum may start at 0), with S[k][j] read from LDS, the
-  // same address in every lane.
-  __syncthreads();
-  {
-    dx4 Sacc = dx4{0, 0, 0, 0};
-    // 16 k-steps of this wave at a time: every LDS read issued (clamped addresses) before
-    // the entries of V are selected and the products run, in k order
-    for (int base = 4 * wave; base < rows; base += 256) {
-      const int a = lane & 15;
-      const int ac = a < w ? a : 0;
-      double vv[16];
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int i = base + 16 * q + (lane >> 4);
-        vv[q] = P[(i < rows ? i : rows - 1) + (size_t)rows * ac];
-      }
-#pragma unroll
-      for (int q = 0; q < 16; q++) {
-        const int i = base + 16 * q + (lane >> 4);
-        vv[q] = (i < rows && a < w) ? (i == a ? 1.0 : (i > a ? vv[q] : 0.0)) : 0.0;
-      }
-#pragma unroll
-      for (int q = 0; q < 16; q++)
-        if (base + 16 * q < rows) Sacc = mfma16(vv[q], vv[q], Sacc);
-    }
-    __syncthreads();   // every wave is done reading P: its space takes the partial sums
-    double* Sred = P;  // [wave][r][lane]
-    double* Sm = P + 16 * 64;   // S, 16 x 16 row-major
-    if (wave > 0) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) Sred[(wave * 4 + r) * 64 + lane] = Sacc[r];
-    }
-    __syncthreads();
-    if (wave == 0) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        Sacc[r] += (Sred[(4 + r) * 64 + lane] + Sred[(8 + r) * 64 + lane]) + Sred[(12 + r) * 64 + lane];
-        Sm[(4 * r + (lane >> 4)) * 16 + (lane & 15)] = Sacc[r];   // element r: S[4r + l/16][l%16]
-      }
-      __builtin_amdgcn_wave_barrier();
-      double row[16];
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < j; k++) acc += row[k] * Sm[k * 16 + j];
-        const double tj = sc[j];
-        double t = 0.0;
-        if (lane < w && j < w) {
-          if (j == lane) t = tj;
-          else if (j > lane) t = -tj * acc;
-        }
-        row[j] = t;
-      }
-      if (lane < 16) {
-#pragma unroll
-        for (int j = 0; j < 16; j++) it.tfac[(size_t)p0 * 16 + lane * 16 + j] = row[j];
-      }
-    }
+  if (tid >= kQrThreads - kQrMaxW) {
+    const int j = tid - (kQrThreads - kQrMaxW);
+    it.tfac[(size_t)p0 * 16 + j] = j < w ? sc[j] : 0.0;
   }
   if (last) {
     // D = |diag R| (0 -> 1) of the earlier panels (final since their launches)
@@ -670,7 +610,8 @@ __device__ __forceinline__ double vpanel(const double* Vp, int n, int rows, int 
 }
 
 __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, int w, int cb, int mode,
-                                              double (&red)[4][4][64], double (&nred)[4][16]) {
+                                              double (&red)[4][4][64], double (&sred)[4][4][64],
+                                              double (&stm)[4][2][256], double (&nred)[4][16]) {
   const int rows = n - p0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -686,16 +627,15 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
   // the first panel's Q strips start from the identity (U is not read; every column of the
   // strip is written)
   const bool ident = mode == 2 && p0 == 0;
-  double ta[4];
+  // the panel's taus (zero beyond w)
+  double tz[16];
 #pragma unroll
-  for (int kc = 0; kc < 4; kc++) {
-    const int kk = 4 * kc + lk;
-    ta[kc] = trans ? it.tfac[(size_t)p0 * 16 + kk * 16 + li] : it.tfac[(size_t)p0 * 16 + li * 16 + kk];
-  }
+  for (int j = 0; j < 16; j++) tz[j] = it.tfac[(size_t)p0 * 16 + j];
   const int nblk = (rows + 15) / 16;
   const int ngroups = (nblk + 4 * kReflRB - 1) / (4 * kReflRB);
   double xb[kReflRB][4], vb[kReflRB][4], va[kReflRB][4];
   dx4 Wacc = dx4{0, 0, 0, 0};
+  dx4 Sacc = dx4{0, 0, 0, 0};   // S = V^T V: vb is both operands
   for (int g = 0; g < ngroups; g++) {
 #pragma unroll
     for (int q = 0; q < kReflRB; q++) {
@@ -720,14 +660,57 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
 #pragma unroll
     for (int q = 0; q < kReflRB; q++)
 #pragma unroll
-      for (int kc = 0; kc < 4; kc++) Wacc = mfma16(vb[q][kc], xb[q][kc], Wacc);
+      for (int kc = 0; kc < 4; kc++) {
+        Wacc = mfma16(vb[q][kc], xb[q][kc], Wacc);
+        Sacc = mfma16(vb[q][kc], vb[q][kc], Sacc);
+      }
   }
 #pragma unroll
-  for (int r = 0; r < 4; r++) red[wave][r][lane] = Wacc[r];
+  for (int r = 0; r < 4; r++) {
+    red[wave][r][lane] = Wacc[r];
+    sred[wave][r][lane] = Sacc[r];
+  }
   __syncthreads();
   dx4 Wt;
 #pragma unroll
   for (int r = 0; r < 4; r++) Wt[r] = (red[0][r][lane] + red[1][r][lane]) + (red[2][r][lane] + red[3][r][lane]);
+  // compact-WY factor T (upper triangular, 16 x 16, zero beyond w): H_1..H_w = I - V T V^T.
+  // Every wave forms it on its own (no workgroup barrier): S row-major into the wave's
+  // space, then lane i builds row i of T from its own earlier entries,
+  // T[i][j] = -tau_j sum_{k=i}^{j-1} T[i][k] S[k][j] (T[i][k] = 0 for k < i, so the sum may
+  // start at 0), with S[k][j] read from LDS, the same address in every lane.
+  double ta[4];
+  {
+    double* Sm = stm[wave][0];
+    double* Tm = stm[wave][1];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      Sm[(4 * r + lk) * 16 + li] = (sred[0][r][lane] + sred[1][r][lane]) + (sred[2][r][lane] + sred[3][r][lane]);
+    __builtin_amdgcn_wave_barrier();
+    double row[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = 0; k < j; k++) acc += row[k] * Sm[k * 16 + j];
+      double t = 0.0;
+      if (lane < w && j < w) {
+        if (j == lane) t = tz[j];
+        else if (j > lane) t = -tz[j] * acc;
+      }
+      row[j] = t;
+    }
+    if (lane < 16) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) Tm[lane * 16 + j] = row[j];
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int kc = 0; kc < 4; kc++) {
+      const int kk = 4 * kc + lk;
+      ta[kc] = trans ? Tm[kk * 16 + li] : Tm[li * 16 + kk];
+    }
+  }
   dx4 Y = dx4{0, 0, 0, 0};
 #pragma unroll
   for (int kc = 0; kc < 4; kc++) Y = mfma16(ta[kc], Wt[kc], Y);
@@ -789,6 +772,8 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
 template <bool TAB>
 __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0, int w, int c0, int nA) {
   __shared__ double red[4][4][64];
+  __shared__ double sred[4][4][64];
+  __shared__ double stm[4][2][256];
   __shared__ double nred[4][16];
   const int bx = blockIdx.x;
   if constexpr (TAB) {
@@ -797,12 +782,12 @@ __global__ __launch_bounds__(256) void udt_reflect_kernel(UdtBatch batch, int p0
     const int n = it.n, c0i = it.p0 + it.w;
     const int nAi = c0i < n ? (n - c0i + 15) / 16 : 0, nQi = (n + 15) / 16;
     if (bx >= nAi + nQi) return;
-    if (bx < nAi) reflect_strip(it, n, it.p0, it.w, c0i + 16 * bx, 0, red, nred);
-    else reflect_strip(it, n, it.p0, it.w, 16 * (bx - nAi), 2, red, nred);
+    if (bx < nAi) reflect_strip(it, n, it.p0, it.w, c0i + 16 * bx, 0, red, sred, stm, nred);
+    else reflect_strip(it, n, it.p0, it.w, 16 * (bx - nAi), 2, red, sred, stm, nred);
     return;
   }
-  if (bx < nA) reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * bx, 0, red, nred);
-  else reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, 16 * (bx - nA), 2, red, nred);
+  if (bx < nA) reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, c0 + 16 * bx, 0, red, sred, stm, nred);
+  else reflect_strip(batch.it[blockIdx.y], batch.n, p0, w, 16 * (bx - nA), 2, red, sred, stm, nred);
 }
 
 // T = D^-1 R (pivoted: column j of R goes to column pivot[j]; dirty: the strict lower part
