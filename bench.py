@@ -387,6 +387,11 @@ def cpu_baseline(pkg, model, par, field, sweeps):
     return sweeps / dt, dt
 
 
+def progress(msg):
+    """One line on stderr as each leg ends (the JSON line on stdout comes at the very end)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -454,6 +459,7 @@ def main():
     total, lat, acc = timed_sweeps(ch, args.steps, args.warmup)
     sps = args.steps / total
     st = ch.stats()
+    progress(f"headline: {sps:.2f} sweeps/s over {args.steps} sweeps")
 
     # ---- per-kernel device time (HIP events on the chain's stream) -------------------------
     ksteps = max(2, args.steps // 4)
@@ -516,22 +522,26 @@ def main():
                     "avg_launch_us": kernels[dom]["avg_us"],
                     "flop_per_launch": kernels[dom]["flop_per_launch"]}
     ch.close()
+    progress(f"kernel times over {ksteps} sweeps")
 
     # ---- mixed precision at config C, beside the FP64 chain on the same field ---------------
     mixed = None
     if not args.skip_mixed and "mixed" in legs:
         mixed = mixed_leg(pkg, args.mixed_sweeps)
+        progress(f"mixed: {mixed.get('status')}")
 
     # ---- batched form: config D, 64 chains in lockstep, one launch per kernel kind ---------
     batched = None
     if args.batch > 1 and "batched" in legs:
         batched = batched_leg(pkg, args.batch, args.batch_sweeps)
         batched["aggregate_vs_one_config_b_chain"] = batched["aggregate_sweeps_per_s"] / sps
+        progress(f"batched: {batched['aggregate_sweeps_per_s']:.1f} sweeps/s")
 
     # ---- serving form: Poisson arrivals through the request loop -----------------------------
     serving = None
     if args.serve > 0 and "serving" in legs:
         serving = serve_leg(pkg, args.serve)
+        progress(f"serving: {serving['wall_s']:.1f} s wall")
 
     # ---- CPU baseline --------------------------------------------------------------------
     cpu = None
@@ -542,8 +552,10 @@ def main():
                          f"after the stack build, {cdt:.1f} s, oracle/dqmc_oracle.c on one host core",
                "host_cores_available": os.cpu_count(),
                "gpu_over_cpu": sps / cps}
+        progress(f"cpu oracle, one core: {cdt:.1f} s")
         if args.cpu_procs > 0:
             cpu["all_cores"] = cpu_all_cores(args.cpu_procs)
+            progress("cpu oracle, all cores")
 
     # algorithmic work of a sweep: b n^3 (12 M + 46 K) + 2 b n^2 A (GEMM 2n^3, TRSM n^3,
     # UDT 3n^3; A accepted flips per sweep), against the FP64 matrix-core peak
