@@ -443,18 +443,26 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
     if (lane < w) slot = p0 + lane;
     else if (lane < 2 * w) slot = cw >= p0 + w ? cw : -1;
     int cont = slot;
-    for (int t = 0; t < w; t++) {
+    int broken = 0;
+    auto swap_step = [&](int t) {
       const int c = __builtin_amdgcn_readlane(selc, t);
       const unsigned long long hold = __ballot(slot >= 0 && cont == c);
       // every choice is held by exactly one tracked slot; an empty ballot means a broken
       // selection: the swap is skipped and the event counted as a device error
-      if (!hold && lane == 0 && it.err) atomicAdd(it.err, 1ull);
+      broken += hold ? 0 : 1;
       const int src = hold ? __ffsll((long long)hold) - 1 : t;  // lane whose slot holds c
       const int a_cont = __builtin_amdgcn_readlane(cont, t);      // content of slot p0+t
       const int b_cont = __builtin_amdgcn_readlane(cont, src);
       if (lane == t) cont = b_cont;
       if (lane == src && src != t) cont = a_cont;
+    };
+    if (w == kQrMaxW) {       // full panels: straight-line steps
+#pragma unroll
+      for (int t = 0; t < kQrMaxW; t++) swap_step(t);
+    } else {
+      for (int t = 0; t < w; t++) swap_step(t);
     }
+    if (broken && lane == 0 && it.err) atomicAdd(it.err, (unsigned long long)broken);
     const bool moved = slot >= 0 && cont != slot;
     const unsigned long long mm = __ballot(moved);
     const int idx = __popcll(mm & ((1ull << lane) - 1ull));
