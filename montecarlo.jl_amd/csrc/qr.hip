@@ -124,7 +124,10 @@ __device__ __forceinline__ bool pivot_before(double a, int ia, double b, int ib)
 
 // Value of lane (lane ^ S).  Inside a 16-lane row with DPP: xor 1 and xor 2 are quad
 // permutations, xor 4 a half-row mirror (i <-> 7 - i) followed by a quad reversal, xor 8
-// a row rotation by 8; across rows through ds_bpermute.
+// a row rotation by 8.  Across rows with the gfx950 swaps of two registers (here two
+// copies of x): permlane16_swap exchanges the odd rows of the first with the even rows of
+// the second, so lanes of odd rows find lane - 16 in the first result and lanes of even
+// rows lane + 16 in the second; permlane32_swap does the same with the two 32-lane halves.
 template <int S>
 __device__ __forceinline__ int lane_xor(int x) {
   if constexpr (S == 1) {
@@ -136,8 +139,13 @@ __device__ __forceinline__ int lane_xor(int x) {
     return __builtin_amdgcn_update_dpp(0, m, 0x1B, 0xF, 0xF, false);    // quad_perm [3,2,1,0]
   } else if constexpr (S == 8) {
     return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, false);   // row_ror 8
+  } else if constexpr (S == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    return (__lane_id() & 16) ? (int)r[0] : (int)r[1];
   } else {
-    return __shfl_xor(x, S, 64);
+    static_assert(S == 32, "lane_xor: strides 1 .. 32");
+    const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (__lane_id() & 32) ? (int)r[0] : (int)r[1];
   }
 }
 
