@@ -191,14 +191,20 @@ def measured_traffic(kernel_tag):
     return None
 
 
-def serve_leg(pkg, nreq, rate=20.0, seed=2024, sweep_cap=2):
+# servers whose requests outlived the serving leg's deadline: kept open (closing one waits for
+# its running requests), and the process leaves with os._exit once its line is printed
+_LEFT_RUNNING = []
+
+
+def serve_leg(pkg, nreq, rate=20.0, seed=2024, sweep_cap=2, max_wall=150.0):
     """Config E's request mix through the serving loop: Poisson arrivals at `rate` per second
     of chains with L uniform in {8..32}, square (N = L^2) or honeycomb (N = 2 L^2) lattice,
     beta uniform in [5, 20] (M = round(10 beta)), U uniform in [-8, 8], dtau = 0.1, FP64.
     The drawn sweep count (uniform in {200..2000}) is capped at `sweep_cap` per request so
     that the leg takes about a minute (a full request runs for up to hours); shapes, beta and
     U are drawn as in the full mix.  Admission by HBM footprint (64 GiB budget), at most 16
-    chains at once; latency is submit -> result (stack build included)."""
+    chains at once; latency is submit -> result (stack build included).  Requests still
+    running `max_wall` seconds after the first submission are reported, not waited for."""
     from montecarlo_jl_amd import _lib as L
     rng = np.random.default_rng(seed)
     srv = L.Server(memory_budget=64 << 30, max_active=16)
@@ -224,16 +230,34 @@ def serve_leg(pkg, nreq, rate=20.0, seed=2024, sweep_cap=2):
             sweeps += ns
             drawn += ns_full
             shapes.append(len(lat))
-        lats, queued = [], []
-        for rid in ids:
-            srv.wait(rid)
-            r = srv.result(rid)
-            lats.append(r["latency_s"])
-            queued.append(r["queued_s"])
+        lats, queued, failed = [], [], 0
+        pending = list(range(len(ids)))
+        while pending and time.perf_counter() - t0 < max_wall:
+            for i in list(pending):
+                st = srv.poll(ids[i])
+                if st not in ("done", "failed"):
+                    continue
+                pending.remove(i)
+                if st == "done":
+                    r = srv.result(ids[i])
+                    lats.append(r["latency_s"])
+                    queued.append(r["queued_s"])
+                else:
+                    failed += 1
+                progress(f"serving: request {i} ({shapes[i]} sites) {st} at "
+                         f"{time.perf_counter() - t0:.1f} s")
+            time.sleep(0.02)
         wall = time.perf_counter() - t0
         peak = srv.status()["peak_active"]
     finally:
-        srv.close()
+        if pending:
+            _LEFT_RUNNING.append(srv)
+        else:
+            srv.close()
+    if pending or failed or not lats:
+        return {"requests": nreq, "status": "incomplete", "completed": len(lats), "failed": failed,
+                "still_running": len(pending), "wall_s": wall, "max_wall_s": max_wall,
+                "still_running_sites": [shapes[i] for i in pending]}
     return {"requests": nreq, "arrival_rate_per_s": rate, "sweeps": sweeps, "wall_s": wall,
             "aggregate_sweeps_per_s": sweeps / wall, "peak_active": peak,
             "latency_ms": {"p50": float(np.percentile(lats, 50)) * 1e3,
@@ -541,7 +565,7 @@ def main():
     serving = None
     if args.serve > 0 and "serving" in legs:
         serving = serve_leg(pkg, args.serve)
-        progress(f"serving: {serving['wall_s']:.1f} s wall")
+        progress(f"serving: {serving['wall_s']:.1f} s wall, {serving.get('status', 'complete')}")
 
     # ---- CPU baseline --------------------------------------------------------------------
     cpu = None
@@ -592,3 +616,8 @@ def main():
 
 if __name__ == "__main__":
     main()
+    if _LEFT_RUNNING:
+        # a server with requests past the deadline: closing it would wait for them
+        sys.stdout.flush()
+        sys.stderr.flush()
+        os._exit(0)
