@@ -149,6 +149,13 @@ __device__ __forceinline__ int lane_xor(int x) {
   }
 }
 
+template <int S>
+__device__ __forceinline__ double lane_xor_d(double x) {
+  const int lo = lane_xor<S>(__double2loint(x));
+  const int hi = lane_xor<S>(__double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+
 template <int SIZE, int STRIDE>
 __device__ __forceinline__ void bitonic_stage(double& kv, int& ki, int lane) {
   const int lo = lane_xor<STRIDE>(__double2loint(kv));
@@ -773,8 +780,8 @@ __device__ __forceinline__ void reflect_strip(const UdtItem& it, int n, int p0, 
     }
   }
   if (mode == 0) {
-    ss += __shfl_xor(ss, 16, 64);
-    ss += __shfl_xor(ss, 32, 64);
+    ss += lane_xor_d<16>(ss);
+    ss += lane_xor_d<32>(ss);
     if (lk == 0) nred[wave][li] = ss;
     __syncthreads();
     if (tid < 16 && cb + tid < n)
