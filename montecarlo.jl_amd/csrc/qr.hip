@@ -31,7 +31,7 @@ namespace dqmc {
 
 constexpr int kQrThreads = 256;
 constexpr int kQrMaxW = 16;
-constexpr int kPanelDoubles = 16 * 1024;  // 128 KB of panel storage
+constexpr int kPanelDoubles = 4 * 64 * 17;  // the column steps' reduction transposes
 
 // Wave64 sum of a double, result in every lane: butterflies inside each 16-lane row
 // with DPP (quad_perm xor 1, xor 2, row_ror 4, row_ror 8), then the four row sums are
@@ -78,7 +78,7 @@ __host__ __device__ static int udt_rows_per_thread(int rows) {
 }
 
 // Panel width for `rows` remaining rows: the panel lives in registers (R rows x w columns
-// per thread, at most 64 values) and in LDS (rows x w <= kPanelDoubles).
+// per thread, at most 64 values).
 __host__ __device__ static int udt_panel_width(int rows) {
   int w = 64 / udt_rows_per_thread(rows);
   if (w > kQrMaxW) w = kQrMaxW;
@@ -184,8 +184,7 @@ __device__ __forceinline__ void bitonic64(double& kv, int& ki, int lane) {
 static size_t udt_panel_lds_bytes(int n) {
   size_t b = 0;
   b += (size_t)n * 8;                 // nrm
-  b += (size_t)kPanelDoubles * 8;     // panel (V for S = V^T V)
-  b += 256 * 8 * 2;                   // Tf, S
+  b += (size_t)kPanelDoubles * 8;     // the factorisation's reduction transposes
   b += 32 * 8;                        // taus, scalars
   b += 2 * (4 * 16 + 16) * 8;         // wave totals, row j (two buffers)
   b += 4 * 16 * 16;                   // top-16 candidates of each wave
@@ -377,10 +376,8 @@ __global__ __launch_bounds__(kQrThreads) void udt_panel_kernel(UdtBatch batch, i
   extern __shared__ __align__(16) unsigned char smem[];
   double* nrm = reinterpret_cast<double*>(smem);
   double* P = nrm + n;
-  double* Tf = P + kPanelDoubles;
-  double* S = Tf + 256;
-  double* sc = S + 256;          // sc[0..15] taus of the panel
-  double* red = P;               // [4][64][17] reduction transpose (P is written afterwards)
+  double* sc = P + kPanelDoubles;   // sc[0..15] taus of the panel
+  double* red = P;               // [4][64][17] reduction transposes
   double* wt = sc + 32;          // [2][4][16] wave totals, by step parity
   double* rowj = wt + 2 * 64;    // [2][16] row j, by step parity
   double* candv = rowj + 32;     // [64] top-16 norms of each wave
