@@ -210,6 +210,7 @@ def serve_leg(pkg, nreq, rate=20.0, seed=2024, sweep_cap=2, max_wall=150.0):
     srv = L.Server(memory_budget=64 << 30, max_active=16)
     t0 = time.perf_counter()
     ids, sweeps, t_next, drawn, shapes = [], 0, 0.0, 0, []
+    pending = []
     try:
         for k in range(nreq):
             t_next += rng.exponential(1.0 / rate)
