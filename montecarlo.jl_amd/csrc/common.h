@@ -152,6 +152,8 @@ hipError_t launch_set_identity(double* A, int n, int nmat, hipStream_t s);
 hipError_t launch_set_identity_f32(float* A, int n, int nmat, hipStream_t s);
 hipError_t launch_fill(double* x, double v, int64_t count, hipStream_t s);
 hipError_t launch_copy(double* dst, const double* src, int64_t count, hipStream_t s);
+hipError_t launch_copy3(double* d0, const double* s0, int64_t n0, double* d1, const double* s1, int64_t n1,
+                        double* d2, const double* s2, int64_t n2, hipStream_t s);
 // field <-> BitArray chunks (64 entries per uint64, least significant bit first)
 hipError_t launch_pack_field(const int8_t* conf, int64_t count, uint64_t* chunks, hipStream_t s);
 hipError_t launch_unpack_field(const uint64_t* chunks, int64_t count, int8_t* conf, hipStream_t s);

@@ -629,6 +629,11 @@ int calculate_greens(Chain* h, double* out) {
 
 int slot_to(Chain* h, int s, double* U, double* D, double* T) {
   const size_t m = (size_t)h->flv * h->nn;
+  if (!h->rec) {   // one launch for the three
+    HIPCHK(h, launch_copy3(U, uslot(h, s, 0), (int64_t)m, T, tslot(h, s, 0), (int64_t)m, D, dslot(h, s, 0),
+                           (int64_t)h->flv * h->n, h->st));
+    return DQMC_OK;
+  }
   CHK(copyd(h, U, uslot(h, s, 0), m));
   CHK(copyd(h, T, tslot(h, s, 0), m));
   return copyd(h, D, dslot(h, s, 0), (size_t)h->flv * h->n);

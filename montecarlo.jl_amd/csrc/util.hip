@@ -86,6 +86,32 @@ hipError_t launch_copy(double* dst, const double* src, int64_t count, hipStream_
   hipLaunchKernelGGL(copy_kernel, grid_for(count), dim3(256), 0, s, dst, src, count);
   return hipGetLastError();
 }
+// Up to three copies in one launch (a stack slot's U, T and D): one grid-stride pass over
+// the segments laid end to end.
+struct CopySegs {
+  double* dst[3];
+  const double* src[3];
+  int64_t count[3];
+};
+__global__ void copy3_kernel(CopySegs c) {
+  const int64_t total = c.count[0] + c.count[1] + c.count[2];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = i < c.count[0] ? 0 : (i < c.count[0] + c.count[1] ? 1 : 2);
+    const int64_t j = k == 0 ? i : (k == 1 ? i - c.count[0] : i - c.count[0] - c.count[1]);
+    c.dst[k][j] = c.src[k][j];
+  }
+}
+hipError_t launch_copy3(double* d0, const double* s0, int64_t n0, double* d1, const double* s1, int64_t n1,
+                        double* d2, const double* s2, int64_t n2, hipStream_t s) {
+  if (n0 < 0 || n1 < 0 || n2 < 0) return hipErrorInvalidValue;
+  CopySegs c{{d0, d1, d2}, {s0, s1, s2}, {n0, n1, n2}};
+  const int64_t total = n0 + n1 + n2;
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy3_kernel, grid_for(total), dim3(256), 0, s, c);
+  return hipGetLastError();
+}
+
 
 __global__ void f64_to_f32_kernel(float* __restrict__ dst, const double* __restrict__ src, int64_t count) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count;
